@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark: 2D2V two-stream instability at 128^4, float64, one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+A step is one Strang-split Vlasov-Poisson step (X, Y drifts dt/2, rho,
+FFT Poisson, Vx, Vy kicks dt, Y, X drifts dt/2) over the whole
+f[128,128,128,128] (2.15 GB, resident in HBM).  Prints ONE JSON line:
+value = phase-space cell-updates/s, plus the HBM roofline of the dominant
+kernel (timed with events on the stream it runs on) and a CPU baseline
+(the NumPy oracle on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "jax-pytorch-vlasov_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+N = 128
+L = 10.0 * math.pi      # k = 0.2
+VMAX = 8.0
+DT = 0.05
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N, help="points per axis (default 128)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(n: int) -> dict:
+    """Time the NumPy oracle on a bounded sample of the same step.
+
+    Sample: every sweep runs on 1/16 of the lines at full line length
+    (X, Y drifts and the Vx kick on f[:, :, :, :n/16], the Vy kick and rho on
+    f[:n/16]), the Poisson solve on the full [n, n] grid; the step time is
+    the sum scaled by 16.
+    """
+    import numpy as np
+    from oracle import vlasov_oracle as vo
+
+    g = vo.grid_2d(n, n, n, n, L, L, VMAX, VMAX)
+    frac = 16
+    m = max(n // frac, 1)
+    rng = np.random.default_rng(0)
+    a = rng.random((n, n, n, m))
+    b = rng.random((m, n, n, n))
+    E = rng.normal(size=(n, n)) * 0.01
+    t = {}
+
+    def timed(key, fn):
+        t0 = time.perf_counter()
+        fn()
+        t[key] = time.perf_counter() - t0
+
+    timed("x", lambda: vo.advect_axis(a, g.vx[None, :, None], 0, DT / 2, g.dx, True))
+    timed("y", lambda: vo.advect_axis(a, g.vy[None, None, :m], 1, DT / 2, g.dy, True))
+    timed("vx", lambda: vo.advect_axis(a, E[:, :, None], 2, DT, g.dvx, False))
+    timed("vy", lambda: vo.advect_axis(b, E[:m, :, None], 3, DT, g.dvy, False))
+    timed("rho", lambda: vo.compute_rho(b, g.dvx, g.dvy))
+    rho = 1.0 + 0.001 * rng.normal(size=(n, n))
+    timed("poisson", lambda: vo.compute_poisson_real(rho, g.kx, g.ky, g.inv_ksq))
+    sample_s = sum(t.values())
+    step_s = (n / m) * (2 * t["x"] + 2 * t["y"] + t["vx"] + t["vy"] + t["rho"]) + t["poisson"]
+    return {
+        "value": n ** 4 / step_s,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"NumPy oracle (float64, single thread) on 1/{n // m} of the lines of every sweep of one "
+                   f"{n}^4 step, full-size Poisson; {sample_s:.1f} s of CPU work, scaled to one step "
+                   f"({step_s:.1f} s/step)"),
+        "cpu": _cpu_model(),
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the solver runs on a single "
+              "MI355X and has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from vlasov_amd import configs, runner
+    from vlasov_amd import vlasov2D2V as v2
+
+    n = args.n
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    grid = v2.Grid.create(n, n, n, n, L, L, VMAX, VMAX, device=dev, dtype=torch.float64)
+    f = configs.two_stream_2d(grid)
+    run = runner.Run2D(grid, f, DT, physics_mode=True)
+    del f
+    for _ in range(args.warmup):
+        run.step()
+    torch.cuda.synchronize()
+
+    names = runner._KERNELS_2D
+    events = {k: [] for k in names}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run.step(events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    ms = {k: sum(e0.elapsed_time(e1) for e0, e1 in v) / max(len(v), 1) for k, v in events.items()}
+    cells = n ** 4
+    t_step = elapsed / args.steps
+    sweep_keys = [k for k in names if k not in ("rho", "poisson")]
+    dom = max(sweep_keys, key=lambda k: ms[k])
+    bytes_per_launch = 2 * 8 * cells  # one read + one write of f per sweep
+    achieved = bytes_per_launch / (ms[dom] * 1e-3) / 1e9
+    model_gbs = 13 * 8 * cells / t_step / 1e9  # reference traffic model (13 f-accesses per step)
+
+    out = {
+        "metric": "phase-space cell-updates/s (2D2V 128^4 two-stream, FP64)",
+        "value": cells / t_step,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (closed-form two-stream initial condition)",
+        "config": {"workload": f"2D2V two-stream instability {n}^4 FP64, k=0.2, v in [-8,8], dt=0.05",
+                   "shape": [n, n, n, n], "parallelism": "single GPU"},
+        "model_GBps_13access": model_gbs,
+        "model_frac_hbm_peak": model_gbs / HBM_PEAK_GBS,
+        "kernel_ms": ms,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

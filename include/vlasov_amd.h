@@ -1,0 +1,126 @@
+/*
+ * vlasov_amd.h -- C ABI of the MI355X (gfx950) Vlasov-Poisson library.
+ *
+ * Plain pointers and sizes only.  Every pointer argument named f / vel / E /
+ * rho / phi / k* / ws is a DEVICE pointer.  Every entry point is
+ * asynchronous on `stream` (a hipStream_t; NULL = the null stream), never
+ * synchronises and never allocates device memory: scratch space is passed
+ * in as `ws`/`ws_bytes`, sized by the matching *_workspace() query.
+ *
+ * Return value: 0 on success, a negative VP_ERR_* code for a rejected
+ * argument (nothing was launched), or a positive hipError_t from the launch.
+ * vp_error_string() turns either into text.
+ *
+ * Array layouts are C order (last index fastest), as in the reference
+ * PyTorch solver:
+ *   1D1V  f[Nx][Nv]                  (simulations/vlasov1d_1v/pytorch/src/vlasov1D1V.py)
+ *   2D2V  f[Nx][Ny][Nvx][Nvy]        (simulations/vlasov2d_2v/pytorch/src/vlasov2D2V.py)
+ * Below, P1 and P2 abbreviate those two files.
+ */
+#ifndef VLASOV_AMD_H
+#define VLASOV_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* vp_stream_t;
+
+enum {
+    VP_OK = 0,
+    VP_ERR_ARG = -1,        /* null pointer, bad axis / flag                 */
+    VP_ERR_SHAPE = -2,      /* size out of the supported range              */
+    VP_ERR_WORKSPACE = -3,  /* ws too small (see the *_workspace query)     */
+    VP_ERR_UNSUPPORTED = -4 /* shape/variant this build has no kernel for   */
+};
+
+const char* vp_error_string(int code);
+int vp_abi_version(void);
+
+/* ---------------------------------------------------------------------
+ * Semi-Lagrangian sweep along one axis of a 4D array.
+ *
+ * Replaces advect_1d (P2:356-413) and advect_1d_vectorized (P1:212-256,
+ * P2:287-353) without the permute/reshape copies: for every line along
+ * `axis`, out[i] = sum_k w_k(alpha_i) * in[(il_i + k - 2)], with
+ * dep_i = i - (vel*dt)/dx, il_i = floor(dep_i), alpha_i = dep_i - il_i and
+ * the 6-point Lagrange weights in the reference's power form
+ * (lagrange5_interp_sum, P1:163-189).  periodic != 0 wraps the taps
+ * (remainder); periodic == 0 zeroes taps outside [0, n) (Dirichlet clamp +
+ * mask, P1:248-253).
+ *
+ * The velocity of element (i0,i1,i2,i3) is vel[sum_d i_d * vel_strides[d]]
+ * (stride 0 = broadcast).  vel_strides[axis] != 0 gives a per-point
+ * velocity, the [B, N] form accepted by advect_1d_vectorized.
+ * A 2D array f[B][N] is shape {1,1,B,N}.
+ * f_in and f_out must not overlap.
+ * --------------------------------------------------------------------- */
+int vp_advect_f64(const double* f_in, double* f_out, const int64_t shape[4], int axis,
+                  const double* vel, const int64_t vel_strides[4],
+                  double dt, double dx, int periodic, vp_stream_t stream);
+
+/* FP32-storage form: f in float32, departure/weights in float64, the
+ * 6-tap sum in float32.  No reference counterpart (the reference's float32
+ * mode computes everything in float32). */
+int vp_advect_f32(const float* f_in, float* f_out, const int64_t shape[4], int axis,
+                  const double* vel, const int64_t vel_strides[4],
+                  double dt, double dx, int periodic, vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Velocity moments and field solves.
+ * --------------------------------------------------------------------- */
+
+/* Row sums with compensated (Neumaier) float64 accumulation:
+ *   out[r] = ((sum_j f[r][j]) * scale1) * scale2 - shift
+ * 1D rho (P1:559-561):  rows=Nx, len=Nv, scale1=dv,  scale2=1, shift=1
+ * 2D rho (P2:416-431):  rows=Nx*Ny, len=Nvx*Nvy, scale1=dvx, scale2=dvy, shift=0 */
+int vp_row_sum_f64(const double* f, int64_t rows, int64_t len, double scale1, double scale2,
+                   double shift, double* out, vp_stream_t stream);
+int vp_row_sum_f32(const float* f, int64_t rows, int64_t len, double scale1, double scale2,
+                   double shift, double* out, vp_stream_t stream);
+
+/* solve_poisson_real (P1:538-572) from rho = n - 1 (already reduced):
+ * E = irfft(-i rho_k inv_kx), phi = irfft(rho_k inv_kx_sq).  nx <= 8192. */
+size_t vp_poisson_1d_workspace(int64_t nx);
+int vp_poisson_1d_f64(const double* rho, int64_t nx, const double* inv_kx,
+                      const double* inv_kx_sq, double* E, double* phi,
+                      void* ws, size_t ws_bytes, vp_stream_t stream);
+
+/* compute_poisson_real (P2:488-517): delta_rho = rho - 1, phi_k =
+ * rfft2(delta_rho) * inv_ksq, E*_k = -i k* phi_k, three irfft2(s=(nx,ny)).
+ * kx is [nx] (fftfreq order), ky is [ny/2+1], inv_ksq is [nx][ny/2+1].
+ * Any output pointer may be NULL.  nx, ny <= 512. */
+size_t vp_poisson_2d_workspace(int64_t nx, int64_t ny);
+int vp_poisson_2d_f64(const double* rho, int64_t nx, int64_t ny, const double* kx,
+                      const double* ky, const double* inv_ksq, double* Ex, double* Ey,
+                      double* delta_rho, double* phi, void* ws, size_t ws_bytes,
+                      vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Full Strang steps (one time step, functional: f_in is not modified).
+ * --------------------------------------------------------------------- */
+
+/* vlasov_step (P1:574-622): X(dt/2), Poisson, V(dt, E), X(dt/2).
+ * E, rho, phi: [nx] outputs (mid-step field); zero when physics_mode == 0. */
+size_t vp_step_1d1v_workspace(int64_t nx, int64_t nv);
+int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
+                     const double* v, const double* inv_kx, const double* inv_kx_sq,
+                     double dx, double dv, int physics_mode, double* E, double* rho,
+                     double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
+
+/* vlasov_step_impl (P2:943-998): X, Y (dt/2), Poisson, Vx, Vy (dt), Y, X (dt/2).
+ * Ex, Ey, delta_rho, phi: [nx][ny] outputs; zero when physics_mode == 0. */
+size_t vp_step_2d2v_workspace(const int64_t shape[4]);
+int vp_step_2d2v_f64(const double* f_in, double* f_out, const int64_t shape[4], double dt,
+                     const double* vx, const double* vy, const double* kx, const double* ky,
+                     const double* inv_ksq, double dx, double dy, double dvx, double dvy,
+                     int physics_mode, double* Ex, double* Ey, double* delta_rho,
+                     double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VLASOV_AMD_H */

@@ -1,0 +1,192 @@
+// Semi-Lagrangian directional sweeps (generic forms).
+//
+// advect_1d (vlasov2D2V.py:356-413) and advect_1d_vectorized
+// (vlasov1D1V.py:212-256) materialise the swept axis last, build int64
+// [B, N, 6] index tensors and gather.  Here the array is viewed in place as
+// [outer][n][inner] (inner = stride of the swept axis) and every line is
+// staged once through LDS:
+//   * strided axis (inner > 1): a tile of n rows x C contiguous columns is
+//     loaded with coalesced row segments, each column is one line;
+//   * contiguous axis (inner == 1): a tile of L whole lines.
+// Each element then reads its 6 taps from LDS.  One HBM read and one HBM
+// write per element.
+#include "vp_common.hpp"
+
+namespace vp {
+
+// Velocity offset of element (outer o, line index r, inner i): the outer
+// and inner flat indices are split back into the array dims they cover.
+struct VelMap {
+    int64_t od[3];
+    int64_t os[3];
+    int no;
+    int64_t id[3];
+    int64_t is[3];
+    int ni;
+    int64_t s_line;
+};
+
+__device__ __forceinline__ int64_t split_offset(int64_t flat, const int64_t* dims,
+                                                const int64_t* strides, int nd) {
+    int64_t off = 0;
+    for (int d = nd - 1; d >= 0; --d) {
+        const int64_t q = flat / dims[d];
+        off += (flat - q * dims[d]) * strides[d];
+        flat = q;
+    }
+    return off;
+}
+
+template <typename T>
+__device__ __forceinline__ void weights_as(double alpha, T w[6]) {
+    double wd[6];
+    lagrange5_weights<double>(alpha, wd);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = static_cast<T>(wd[k]);
+}
+
+template <typename T, bool PERIODIC>
+__device__ __forceinline__ T interp_from_lds(const T* tile, int64_t il, double alpha, int n,
+                                             int row_stride, int col) {
+    T w[6];
+    weights_as<T>(alpha, w);
+    T v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int64_t j = il + (k - 2);
+        if (PERIODIC) {
+            v[k] = tile[wrap_index(j, n) * row_stride + col];
+        } else {
+            const int jc = static_cast<int>(j < 0 ? 0 : (j > n - 1 ? n - 1 : j));
+            const T valid = (j >= 0 && j < n) ? T(1) : T(0);
+            v[k] = tile[jc * row_stride + col] * valid;
+        }
+    }
+    return tap_sum<T>(w, v);
+}
+
+// Tile = n rows (the swept axis) x C columns (contiguous inner elements).
+template <typename T, bool PERIODIC>
+__global__ __launch_bounds__(256) void advect_strided_kernel(
+    const T* __restrict__ fin, T* __restrict__ fout, int n, int64_t inner, int C,
+    int64_t n_chunks, const double* __restrict__ vel, VelMap vm, double dt, double dx) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* tile = reinterpret_cast<T*>(smem_raw);
+    const int64_t blk = blockIdx.x;
+    const int64_t o = blk / n_chunks;
+    const int64_t i0 = (blk - o * n_chunks) * C;
+    const int c = threadIdx.x % C;
+    const int r0 = threadIdx.x / C;
+    const int rstep = blockDim.x / C;
+    const bool cval = (i0 + c) < inner;
+    const int64_t base = o * static_cast<int64_t>(n) * inner + i0 + c;
+    if (cval)
+        for (int r = r0; r < n; r += rstep) tile[r * C + c] = fin[base + r * inner];
+    __syncthreads();
+    if (!cval) return;
+    const int64_t voff = split_offset(o, vm.od, vm.os, vm.no) + split_offset(i0 + c, vm.id, vm.is, vm.ni);
+    for (int r = r0; r < n; r += rstep) {
+        const Departure d = departure(r, vel[voff + r * vm.s_line], dt, dx);
+        fout[base + r * inner] = interp_from_lds<T, PERIODIC>(tile, d.il, d.alpha, n, C, c);
+    }
+}
+
+// Tile = L whole contiguous lines of length n; wave w handles lines w, w+4, ...
+template <typename T, bool PERIODIC>
+__global__ __launch_bounds__(256) void advect_contig_kernel(
+    const T* __restrict__ fin, T* __restrict__ fout, int64_t n_lines, int n, int L,
+    const double* __restrict__ vel, VelMap vm, double dt, double dx) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* tile = reinterpret_cast<T*>(smem_raw);
+    const int64_t l0 = static_cast<int64_t>(blockIdx.x) * L;
+    const int64_t nl = (n_lines - l0) < L ? (n_lines - l0) : L;
+    const int64_t cnt = nl * n;
+    const T* src = fin + l0 * n;
+    for (int64_t idx = threadIdx.x; idx < cnt; idx += blockDim.x) tile[idx] = src[idx];
+    __syncthreads();
+    const int lane = threadIdx.x % kWave;
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    for (int l = wid; l < nl; l += nw) {
+        const int64_t voff = split_offset(l0 + l, vm.od, vm.os, vm.no);
+        const T* line = tile + static_cast<int64_t>(l) * n;
+        T* dst = fout + (l0 + l) * n;
+        for (int j = lane; j < n; j += kWave) {
+            const Departure d = departure(j, vel[voff + j * vm.s_line], dt, dx);
+            dst[j] = interp_from_lds<T, PERIODIC>(line, d.il, d.alpha, n, 1, 0);
+        }
+    }
+}
+
+template <typename T>
+static int launch_advect(const T* fin, T* fout, const int64_t shape[4], int axis, const double* vel,
+                         const int64_t vs[4], double dt, double dx, int periodic, hipStream_t s) {
+    VP_RETURN_IF(!fin || !fout || !shape || !vel || !vs, VP_ERR_ARG);
+    VP_RETURN_IF(axis < 0 || axis > 3, VP_ERR_ARG);
+    for (int d = 0; d < 4; ++d) VP_RETURN_IF(shape[d] < 1, VP_ERR_SHAPE);
+    const int64_t n64 = shape[axis];
+    VP_RETURN_IF(n64 > 8192, VP_ERR_UNSUPPORTED);
+    const int n = static_cast<int>(n64);
+    VelMap vm{};
+    int64_t outer = 1, inner = 1;
+    for (int d = 0; d < axis; ++d) {
+        vm.od[vm.no] = shape[d];
+        vm.os[vm.no] = vs[d];
+        ++vm.no;
+        outer *= shape[d];
+    }
+    for (int d = axis + 1; d < 4; ++d) {
+        vm.id[vm.ni] = shape[d];
+        vm.is[vm.ni] = vs[d];
+        ++vm.ni;
+        inner *= shape[d];
+    }
+    vm.s_line = vs[axis];
+    const int block = 256;
+    if (inner > 1) {
+        int C = 64;
+        while (C > 1 && C / 2 >= inner) C /= 2;
+        while (C > 1 && static_cast<int64_t>(n) * C * sizeof(T) > 65536) C /= 2;
+        VP_RETURN_IF(static_cast<int64_t>(n) * C * sizeof(T) > 65536, VP_ERR_UNSUPPORTED);
+        const int64_t chunks = (inner + C - 1) / C;
+        const int64_t blocks = outer * chunks;
+        VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
+        const size_t lds = static_cast<size_t>(n) * C * sizeof(T);
+        if (periodic)
+            hipLaunchKernelGGL((advect_strided_kernel<T, true>), dim3(blocks), dim3(block), lds, s, fin,
+                               fout, n, inner, C, chunks, vel, vm, dt, dx);
+        else
+            hipLaunchKernelGGL((advect_strided_kernel<T, false>), dim3(blocks), dim3(block), lds, s, fin,
+                               fout, n, inner, C, chunks, vel, vm, dt, dx);
+    } else {
+        int L = n >= 4096 ? 1 : static_cast<int>(4096 / n);
+        if (L > outer) L = static_cast<int>(outer);
+        const int64_t blocks = (outer + L - 1) / L;
+        VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
+        const size_t lds = static_cast<size_t>(L) * n * sizeof(T);
+        if (periodic)
+            hipLaunchKernelGGL((advect_contig_kernel<T, true>), dim3(blocks), dim3(block), lds, s, fin, fout,
+                               outer, n, L, vel, vm, dt, dx);
+        else
+            hipLaunchKernelGGL((advect_contig_kernel<T, false>), dim3(blocks), dim3(block), lds, s, fin, fout,
+                               outer, n, L, vel, vm, dt, dx);
+    }
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+}  // namespace vp
+
+extern "C" int vp_advect_f64(const double* f_in, double* f_out, const int64_t shape[4], int axis,
+                             const double* vel, const int64_t vel_strides[4], double dt, double dx,
+                             int periodic, vp_stream_t stream) {
+    return vp::launch_advect<double>(f_in, f_out, shape, axis, vel, vel_strides, dt, dx, periodic,
+                                     static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_advect_f32(const float* f_in, float* f_out, const int64_t shape[4], int axis,
+                             const double* vel, const int64_t vel_strides[4], double dt, double dx,
+                             int periodic, vp_stream_t stream) {
+    return vp::launch_advect<float>(f_in, f_out, shape, axis, vel, vel_strides, dt, dx, periodic,
+                                    static_cast<hipStream_t>(stream));
+}

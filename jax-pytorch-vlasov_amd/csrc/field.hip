@@ -1,0 +1,275 @@
+// Velocity moments (charge density) and the FFT Poisson / E-field solves.
+//
+// rho: one workgroup per spatial point sums its velocity row with Neumaier
+// compensation (wave shuffle + LDS tree), replacing torch.sum(f, dim=...)
+// of vlasov1D1V.py:559 and vlasov2D2V.py:416-431.
+//
+// Poisson: the spatial grids are small (<= 512 per axis, 128 in the
+// benchmark configs), so the transforms are exact-twiddle float64 DFTs,
+// one line per workgroup, twiddles from sincospi.  Conventions follow
+// torch.fft / numpy.fft: rfft, then c2c along x; the inverse is c2c along x
+// (scaled 1/nx), then c2r along y (scaled 1/ny) reading only the real parts
+// of the DC and Nyquist bins.
+#include "vp_common.hpp"
+
+namespace vp {
+
+template <typename T>
+__global__ __launch_bounds__(256) void row_sum_kernel(const T* __restrict__ f, int64_t len, double s1,
+                                                      double s2, double shift, double* __restrict__ out) {
+    __shared__ double scratch[2 * 256 / kWave];
+    const T* row = f + static_cast<int64_t>(blockIdx.x) * len;
+    KSum k{0.0, 0.0};
+    for (int64_t j = threadIdx.x; j < len; j += blockDim.x) ksum_add(k, static_cast<double>(row[j]));
+    const double total = block_ksum(k, scratch);
+    if (threadIdx.x == 0) out[blockIdx.x] = ((total * s1) * s2) - shift;
+}
+
+// Twiddle table tw[m] = (cos(2 pi m / n), sin(2 pi m / n)).
+__device__ __forceinline__ void fill_twiddles(double2* tw, int n) {
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        double s, c;
+        sincospi(2.0 * static_cast<double>(m) / static_cast<double>(n), &s, &c);
+        tw[m] = make_double2(c, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1D: solve_poisson_real (vlasov1D1V.py:538-572), single workgroup.
+// ws: tw[nx], Ek[nx/2+1], Pk[nx/2+1]  (double2)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restrict__ rho, int nx,
+                                                         const double* __restrict__ inv_kx,
+                                                         const double* __restrict__ inv_kx_sq,
+                                                         double* __restrict__ E, double* __restrict__ phi,
+                                                         double2* __restrict__ ws) {
+    double2* tw = ws;
+    double2* Ek = ws + nx;
+    double2* Pk = Ek + (nx / 2 + 1);
+    fill_twiddles(tw, nx);
+    __syncthreads();
+    const int nk = nx / 2 + 1;
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int x = 0; x < nx; ++x) {
+            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            re += rho[x] * t.x;
+            im -= rho[x] * t.y;
+        }
+        // E_k = -i rho_k inv_kx ; phi_k = rho_k inv_kx^2   (vlasov1D1V.py:556-567)
+        Ek[k] = make_double2(im * inv_kx[k], -re * inv_kx[k]);
+        Pk[k] = make_double2(re * inv_kx_sq[k], im * inv_kx_sq[k]);
+    }
+    __syncthreads();
+    const int khalf = (nx + 1) / 2;  // bins 1..khalf-1 are doubled
+    const double inv_n = 1.0 / static_cast<double>(nx);
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        double se = Ek[0].x, sp = Pk[0].x;
+        for (int k = 1; k < khalf; ++k) {
+            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            se += 2.0 * (Ek[k].x * t.x - Ek[k].y * t.y);
+            sp += 2.0 * (Pk[k].x * t.x - Pk[k].y * t.y);
+        }
+        if ((nx & 1) == 0) {
+            const double sgn = (x & 1) ? -1.0 : 1.0;
+            se += Ek[nx / 2].x * sgn;
+            sp += Pk[nx / 2].x * sgn;
+        }
+        E[x] = se * inv_n;
+        phi[x] = sp * inv_n;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2D: compute_poisson_real (vlasov2D2V.py:488-517).
+// ws: R[nx][nky], T[3][nx][nky]  (double2), nky = ny/2 + 1
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ rho, int nx, int ny,
+                                                    double* __restrict__ delta_rho, double2* __restrict__ R) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* tw = reinterpret_cast<double2*>(smem_raw);
+    double* row = reinterpret_cast<double*>(tw + ny);
+    const int x = blockIdx.x;
+    fill_twiddles(tw, ny);
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) {
+        const double d = rho[static_cast<int64_t>(x) * ny + y] - 1.0;
+        row[y] = d;
+        if (delta_rho) delta_rho[static_cast<int64_t>(x) * ny + y] = d;
+    }
+    __syncthreads();
+    const int nky = ny / 2 + 1;
+    for (int k = threadIdx.x; k < nky; k += blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int y = 0; y < ny; ++y) {
+            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * y) % ny)];
+            re += row[y] * t.x;
+            im -= row[y] * t.y;
+        }
+        R[static_cast<int64_t>(x) * nky + k] = make_double2(re, im);
+    }
+}
+
+__global__ __launch_bounds__(256) void p2d_cols(const double2* __restrict__ R, int nx, int ny,
+                                                const double* __restrict__ kx, const double* __restrict__ ky,
+                                                const double* __restrict__ inv_ksq, double2* __restrict__ T) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* tw = reinterpret_cast<double2*>(smem_raw);
+    double2* col = tw + nx;
+    double2* Q = col + nx;  // [3][nx]
+    const int nky = ny / 2 + 1;
+    const int ky_i = blockIdx.x;
+    fill_twiddles(tw, nx);
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) col[x] = R[static_cast<int64_t>(x) * nky + ky_i];
+    __syncthreads();
+    const double kyv = ky[ky_i];
+    for (int k = threadIdx.x; k < nx; k += blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int x = 0; x < nx; ++x) {
+            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            // (a + ib)(c - is) = (ac + bs) + i(bc - as)
+            re += col[x].x * t.x + col[x].y * t.y;
+            im += col[x].y * t.x - col[x].x * t.y;
+        }
+        const double iks = inv_ksq[static_cast<int64_t>(k) * nky + ky_i];
+        const double pr = re * iks, pi = im * iks;
+        const double kxv = kx[k];
+        Q[k] = make_double2(pr, pi);                      // phi_k
+        Q[nx + k] = make_double2(pi * kxv, -pr * kxv);    // Ex_k = -i kx phi_k
+        Q[2 * nx + k] = make_double2(pi * kyv, -pr * kyv);  // Ey_k = -i ky phi_k
+    }
+    __syncthreads();
+    const double inv_n = 1.0 / static_cast<double>(nx);
+    for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
+        const int q = idx / nx;
+        const int x = idx - q * nx;
+        const double2* Qq = Q + q * nx;
+        double re = 0.0, im = 0.0;
+        for (int k = 0; k < nx; ++k) {
+            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            // (a + ib)(c + is) = (ac - bs) + i(bc + as)
+            re += Qq[k].x * t.x - Qq[k].y * t.y;
+            im += Qq[k].y * t.x + Qq[k].x * t.y;
+        }
+        T[(static_cast<int64_t>(q) * nx + x) * nky + ky_i] = make_double2(re * inv_n, im * inv_n);
+    }
+}
+
+__global__ __launch_bounds__(256) void p2d_rows_inv(const double2* __restrict__ T, int nx, int ny,
+                                                    double* __restrict__ phi, double* __restrict__ Ex,
+                                                    double* __restrict__ Ey) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* tw = reinterpret_cast<double2*>(smem_raw);
+    double2* row = tw + ny;  // [nky]
+    const int nky = ny / 2 + 1;
+    const int x = blockIdx.x % nx;
+    const int q = blockIdx.x / nx;
+    double* out = q == 0 ? phi : (q == 1 ? Ex : Ey);
+    if (!out) return;
+    fill_twiddles(tw, ny);
+    for (int k = threadIdx.x; k < nky; k += blockDim.x)
+        row[k] = T[(static_cast<int64_t>(q) * nx + x) * nky + k];
+    __syncthreads();
+    const int khalf = (ny + 1) / 2;
+    const double inv_n = 1.0 / static_cast<double>(ny);
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) {
+        double s = row[0].x;
+        for (int k = 1; k < khalf; ++k) {
+            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * y) % ny)];
+            s += 2.0 * (row[k].x * t.x - row[k].y * t.y);
+        }
+        if ((ny & 1) == 0) s += row[ny / 2].x * ((y & 1) ? -1.0 : 1.0);
+        out[static_cast<int64_t>(x) * ny + y] = s * inv_n;
+    }
+}
+
+__global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int fill(double* p, int64_t n, double v, hipStream_t s) {
+    if (!p || n <= 0) return VP_OK;
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+template <typename T>
+int row_sum(const T* f, int64_t rows, int64_t len, double s1, double s2, double shift, double* out,
+            hipStream_t s) {
+    VP_RETURN_IF(!f || !out, VP_ERR_ARG);
+    VP_RETURN_IF(rows < 1 || len < 1 || rows > 0x7fffffffLL, VP_ERR_SHAPE);
+    hipLaunchKernelGGL((row_sum_kernel<T>), dim3(rows), dim3(256), 0, s, f, len, s1, s2, shift, out);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+template int row_sum<double>(const double*, int64_t, int64_t, double, double, double, double*, hipStream_t);
+template int row_sum<float>(const float*, int64_t, int64_t, double, double, double, double*, hipStream_t);
+
+size_t poisson1d_ws(int64_t nx) { return static_cast<size_t>(nx + 2 * (nx / 2 + 1)) * sizeof(double2); }
+
+int poisson1d(const double* rho, int64_t nx, const double* inv_kx, const double* inv_kx_sq, double* E,
+              double* phi, void* ws, size_t ws_bytes, hipStream_t s) {
+    VP_RETURN_IF(!rho || !inv_kx || !inv_kx_sq || !E || !phi || !ws, VP_ERR_ARG);
+    VP_RETURN_IF(nx < 1 || nx > 8192, VP_ERR_SHAPE);
+    VP_RETURN_IF(ws_bytes < poisson1d_ws(nx), VP_ERR_WORKSPACE);
+    hipLaunchKernelGGL(poisson1d_kernel, dim3(1), dim3(1024), 0, s, rho, static_cast<int>(nx), inv_kx,
+                       inv_kx_sq, E, phi, static_cast<double2*>(ws));
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+size_t poisson2d_ws(int64_t nx, int64_t ny) {
+    return static_cast<size_t>(4) * nx * (ny / 2 + 1) * sizeof(double2);
+}
+
+int poisson2d(const double* rho, int64_t nx, int64_t ny, const double* kx, const double* ky,
+              const double* inv_ksq, double* Ex, double* Ey, double* delta_rho, double* phi, void* ws,
+              size_t ws_bytes, hipStream_t s) {
+    VP_RETURN_IF(!rho || !kx || !ky || !inv_ksq || !ws, VP_ERR_ARG);
+    VP_RETURN_IF(nx < 1 || ny < 1 || nx > 512 || ny > 512, VP_ERR_SHAPE);
+    VP_RETURN_IF(ws_bytes < poisson2d_ws(nx, ny), VP_ERR_WORKSPACE);
+    const int nxi = static_cast<int>(nx), nyi = static_cast<int>(ny);
+    const int nky = nyi / 2 + 1;
+    double2* R = static_cast<double2*>(ws);
+    double2* T = R + static_cast<int64_t>(nx) * nky;
+    hipLaunchKernelGGL(p2d_rows_fwd, dim3(nxi), dim3(256), ny * (sizeof(double2) + sizeof(double)), s, rho,
+                       nxi, nyi, delta_rho, R);
+    VP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(p2d_cols, dim3(nky), dim3(256), 5 * nx * sizeof(double2), s, R, nxi, nyi, kx, ky,
+                       inv_ksq, T);
+    VP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(p2d_rows_inv, dim3(3 * nxi), dim3(256), (ny + nky) * sizeof(double2), s, T, nxi, nyi,
+                       phi, Ex, Ey);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+}  // namespace vp
+
+extern "C" int vp_row_sum_f64(const double* f, int64_t rows, int64_t len, double scale1, double scale2,
+                              double shift, double* out, vp_stream_t stream) {
+    return vp::row_sum<double>(f, rows, len, scale1, scale2, shift, out, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_row_sum_f32(const float* f, int64_t rows, int64_t len, double scale1, double scale2,
+                              double shift, double* out, vp_stream_t stream) {
+    return vp::row_sum<float>(f, rows, len, scale1, scale2, shift, out, static_cast<hipStream_t>(stream));
+}
+
+extern "C" size_t vp_poisson_1d_workspace(int64_t nx) { return vp::poisson1d_ws(nx); }
+
+extern "C" int vp_poisson_1d_f64(const double* rho, int64_t nx, const double* inv_kx, const double* inv_kx_sq,
+                                 double* E, double* phi, void* ws, size_t ws_bytes, vp_stream_t stream) {
+    return vp::poisson1d(rho, nx, inv_kx, inv_kx_sq, E, phi, ws, ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+extern "C" size_t vp_poisson_2d_workspace(int64_t nx, int64_t ny) { return vp::poisson2d_ws(nx, ny); }
+
+extern "C" int vp_poisson_2d_f64(const double* rho, int64_t nx, int64_t ny, const double* kx, const double* ky,
+                                 const double* inv_ksq, double* Ex, double* Ey, double* delta_rho, double* phi,
+                                 void* ws, size_t ws_bytes, vp_stream_t stream) {
+    return vp::poisson2d(rho, nx, ny, kx, ky, inv_ksq, Ex, Ey, delta_rho, phi, ws, ws_bytes,
+                         static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,133 @@
+// Full Strang steps composed from the sweep and field kernels, plus the
+// small C-ABI utilities (error strings, version).
+#include "vp_common.hpp"
+
+namespace vp {
+int fill(double* p, int64_t n, double v, hipStream_t s);
+template <typename T>
+int row_sum(const T* f, int64_t rows, int64_t len, double s1, double s2, double shift, double* out,
+            hipStream_t s);
+size_t poisson1d_ws(int64_t nx);
+int poisson1d(const double* rho, int64_t nx, const double* inv_kx, const double* inv_kx_sq, double* E,
+              double* phi, void* ws, size_t ws_bytes, hipStream_t s);
+size_t poisson2d_ws(int64_t nx, int64_t ny);
+int poisson2d(const double* rho, int64_t nx, int64_t ny, const double* kx, const double* ky,
+              const double* inv_ksq, double* Ex, double* Ey, double* delta_rho, double* phi, void* ws,
+              size_t ws_bytes, hipStream_t s);
+
+static inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+#define VP_TRY(expr)              \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != VP_OK) return rc_; \
+    } while (0)
+
+}  // namespace vp
+
+extern "C" const char* vp_error_string(int code) {
+    switch (code) {
+        case VP_OK: return "ok";
+        case VP_ERR_ARG: return "invalid argument (null pointer, axis or flag)";
+        case VP_ERR_SHAPE: return "array size outside the supported range";
+        case VP_ERR_WORKSPACE: return "workspace too small";
+        case VP_ERR_UNSUPPORTED: return "shape or variant not supported by this build";
+        default: break;
+    }
+    if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));
+    return "unknown error";
+}
+
+extern "C" int vp_abi_version(void) { return 1; }
+
+// ---------------------------------------------------------------------------
+// 1D1V: vlasov_step (vlasov1D1V.py:574-622)
+// ws: [nx*nv] f scratch | [nx] rho scratch | poisson ws
+// ---------------------------------------------------------------------------
+extern "C" size_t vp_step_1d1v_workspace(int64_t nx, int64_t nv) {
+    return vp::align256(static_cast<size_t>(nx) * nv * sizeof(double)) +
+           vp::align256(static_cast<size_t>(nx) * sizeof(double)) + vp::poisson1d_ws(nx);
+}
+
+extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
+                                const double* v, const double* inv_kx, const double* inv_kx_sq, double dx,
+                                double dv, int physics_mode, double* E, double* rho, double* phi, void* ws,
+                                size_t ws_bytes, vp_stream_t stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VP_RETURN_IF(!f_in || !f_out || !v || !E || !rho || !phi || !ws, VP_ERR_ARG);
+    VP_RETURN_IF(physics_mode && (!inv_kx || !inv_kx_sq), VP_ERR_ARG);
+    VP_RETURN_IF(ws_bytes < vp_step_1d1v_workspace(nx, nv), VP_ERR_WORKSPACE);
+    unsigned char* wb = static_cast<unsigned char*>(ws);
+    double* tmp = reinterpret_cast<double*>(wb);
+    wb += vp::align256(static_cast<size_t>(nx) * nv * sizeof(double));
+    wb += vp::align256(static_cast<size_t>(nx) * sizeof(double));
+    void* pws = wb;
+    const int64_t shape[4] = {1, 1, nx, nv};
+    const int64_t vs_x[4] = {0, 0, 0, 1};  // drift along x with velocity v[j]
+    const int64_t vs_v[4] = {0, 0, 1, 0};  // kick along v with field E[i]
+    const double half = dt / 2.0;
+    VP_TRY(vp_advect_f64(f_in, f_out, shape, 2, v, vs_x, half, dx, 1, stream));
+    if (physics_mode) {
+        VP_TRY(vp::row_sum<double>(f_out, nx, nv, dv, 1.0, 1.0, rho, s));
+        VP_TRY(vp::poisson1d(rho, nx, inv_kx, inv_kx_sq, E, phi, pws, vp::poisson1d_ws(nx), s));
+    } else {
+        VP_TRY(vp::fill(E, nx, 0.0, s));
+        VP_TRY(vp::fill(rho, nx, 0.0, s));
+        VP_TRY(vp::fill(phi, nx, 0.0, s));
+    }
+    VP_TRY(vp_advect_f64(f_out, tmp, shape, 3, E, vs_v, dt, dv, 0, stream));
+    VP_TRY(vp_advect_f64(tmp, f_out, shape, 2, v, vs_x, half, dx, 1, stream));
+    return VP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// 2D2V: vlasov_step_impl (vlasov2D2V.py:943-998)
+// ws: [N^4] f scratch | [nx*ny] rho | poisson ws
+// ---------------------------------------------------------------------------
+extern "C" size_t vp_step_2d2v_workspace(const int64_t shape[4]) {
+    const size_t cells = static_cast<size_t>(shape[0]) * shape[1] * shape[2] * shape[3];
+    return vp::align256(cells * sizeof(double)) +
+           vp::align256(static_cast<size_t>(shape[0]) * shape[1] * sizeof(double)) +
+           vp::poisson2d_ws(shape[0], shape[1]);
+}
+
+extern "C" int vp_step_2d2v_f64(const double* f_in, double* f_out, const int64_t shape[4], double dt,
+                                const double* vx, const double* vy, const double* kx, const double* ky,
+                                const double* inv_ksq, double dx, double dy, double dvx, double dvy,
+                                int physics_mode, double* Ex, double* Ey, double* delta_rho, double* phi,
+                                void* ws, size_t ws_bytes, vp_stream_t stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VP_RETURN_IF(!f_in || !f_out || !shape || !vx || !vy || !Ex || !Ey || !delta_rho || !phi || !ws,
+                 VP_ERR_ARG);
+    VP_RETURN_IF(physics_mode && (!kx || !ky || !inv_ksq), VP_ERR_ARG);
+    VP_RETURN_IF(ws_bytes < vp_step_2d2v_workspace(shape), VP_ERR_WORKSPACE);
+    const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
+    const size_t cells = static_cast<size_t>(nx) * ny * nvx * nvy;
+    unsigned char* wb = static_cast<unsigned char*>(ws);
+    double* tmp = reinterpret_cast<double*>(wb);
+    wb += vp::align256(cells * sizeof(double));
+    double* rho = reinterpret_cast<double*>(wb);
+    wb += vp::align256(static_cast<size_t>(nx) * ny * sizeof(double));
+    void* pws = wb;
+    const double half = dt / 2.0;
+    const int64_t vs_x[4] = {0, 0, 1, 0};  // vx[None, :, None]
+    const int64_t vs_y[4] = {0, 0, 0, 1};  // vy[None, None, :]
+    const int64_t vs_e[4] = {ny, 1, 0, 0};  // E[:, :, None]
+    VP_TRY(vp_advect_f64(f_in, tmp, shape, 0, vx, vs_x, half, dx, 1, stream));
+    VP_TRY(vp_advect_f64(tmp, f_out, shape, 1, vy, vs_y, half, dy, 1, stream));
+    if (physics_mode) {
+        VP_TRY(vp::row_sum<double>(f_out, nx * ny, nvx * nvy, dvx, dvy, 0.0, rho, s));
+        VP_TRY(vp::poisson2d(rho, nx, ny, kx, ky, inv_ksq, Ex, Ey, delta_rho, phi, pws,
+                             vp::poisson2d_ws(nx, ny), s));
+    } else {
+        VP_TRY(vp::fill(Ex, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(Ey, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(delta_rho, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(phi, nx * ny, 0.0, s));
+    }
+    VP_TRY(vp_advect_f64(f_out, tmp, shape, 2, Ex, vs_e, dt, dvx, 0, stream));
+    VP_TRY(vp_advect_f64(tmp, f_out, shape, 3, Ey, vs_e, dt, dvy, 0, stream));
+    VP_TRY(vp_advect_f64(f_out, tmp, shape, 1, vy, vs_y, half, dy, 1, stream));
+    VP_TRY(vp_advect_f64(tmp, f_out, shape, 0, vx, vs_x, half, dx, 1, stream));
+    return VP_OK;
+}

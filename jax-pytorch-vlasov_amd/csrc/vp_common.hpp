@@ -1,0 +1,132 @@
+// Shared device helpers: error plumbing, the 6-point Lagrange weights in
+// the reference's power form, departure-point arithmetic and compensated
+// (Neumaier) float64 accumulation.
+//
+// The whole library is compiled with -ffp-contract=off: every a*b+c below
+// rounds twice, exactly like PyTorch-CPU eager evaluates the reference's
+// expressions, so a sweep reproduces the reference element for element.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vlasov_amd.h"
+
+#define VP_CHECK_LAUNCH()                                  \
+    do {                                                   \
+        hipError_t e_ = hipGetLastError();                 \
+        if (e_ != hipSuccess) return static_cast<int>(e_); \
+    } while (0)
+
+#define VP_RETURN_IF(cond, code) \
+    do {                         \
+        if (cond) return (code); \
+    } while (0)
+
+namespace vp {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Lagrange weights on nodes -2..3 (lagrange5_interp_sum, vlasov1D1V.py:163-189;
+// lagrange5_weights, vlasov1D1V.py:191-210).  Same divisions by literal
+// constants and the same left-to-right association as the reference, so the
+// float64 results are bit-identical to PyTorch-CPU eager.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void lagrange5_weights(T a, T w[6]) {
+    const T a2 = a * a;
+    const T a3 = a2 * a;
+    const T a4 = a3 * a;
+    const T a5 = a4 * a;
+    w[0] = (a / T(20)) - (a2 / T(24)) - (a3 / T(24)) + (a4 / T(24)) - (a5 / T(120));
+    w[1] = -(a / T(2)) + (T(2) * a2 / T(3)) - (a3 / T(24)) - (a4 / T(6)) + (a5 / T(24));
+    w[2] = T(1) - (a / T(3)) - (T(5) * a2 / T(4)) + (T(5) * a3 / T(12)) + (a4 / T(4)) - (a5 / T(12));
+    w[3] = a + (T(2) * a2 / T(3)) - (T(7) * a3 / T(12)) - (a4 / T(6)) + (a5 / T(12));
+    w[4] = -(a / T(4)) - (a2 / T(24)) + (T(7) * a3 / T(24)) + (a4 / T(24)) - (a5 / T(24));
+    w[5] = (a / T(30)) - (a3 / T(24)) + (a5 / T(120));
+}
+
+// Departure point of grid index n for velocity vel:
+// dep = n - (vel*dt)/dx ; il = floor(dep) ; alpha = dep - il
+// (advect_1d_vectorized, vlasov1D1V.py:229-238).
+struct Departure {
+    int64_t il;
+    double alpha;
+};
+
+__device__ __forceinline__ Departure departure(int n, double vel, double dt, double dx) {
+    const double disp = (vel * dt) / dx;
+    const double dep = static_cast<double>(n) - disp;
+    const double fl = floor(dep);
+    Departure d;
+    d.alpha = dep - fl;
+    d.il = static_cast<int64_t>(fl);
+    return d;
+}
+
+__device__ __forceinline__ int wrap_index(int64_t j, int n) {
+    int64_t r = j % n;
+    return static_cast<int>(r < 0 ? r + n : r);
+}
+
+// 6-tap sum w0*f0 + ... + w5*f5, left to right (vlasov1D1V.py:182-189).
+template <typename T>
+__device__ __forceinline__ T tap_sum(const T w[6], const T v[6]) {
+    return w[0] * v[0] + w[1] * v[1] + w[2] * v[2] + w[3] * v[3] + w[4] * v[4] + w[5] * v[5];
+}
+
+// ---------------------------------------------------------------------------
+// Neumaier compensated summation.
+// ---------------------------------------------------------------------------
+struct KSum {
+    double s;
+    double c;
+};
+
+__device__ __forceinline__ void ksum_add(KSum& k, double x) {
+    const double t = k.s + x;
+    if (fabs(k.s) >= fabs(x))
+        k.c += (k.s - t) + x;
+    else
+        k.c += (x - t) + k.s;
+    k.s = t;
+}
+
+__device__ __forceinline__ void ksum_merge(KSum& a, const KSum& b) {
+    ksum_add(a, b.s);
+    a.c += b.c;
+}
+
+__device__ __forceinline__ KSum wave_ksum(KSum k) {
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        KSum o;
+        o.s = __shfl_xor(k.s, off, kWave);
+        o.c = __shfl_xor(k.c, off, kWave);
+        ksum_merge(k, o);
+    }
+    return k;
+}
+
+// Block-wide compensated sum; every thread gets the result.  `scratch` must
+// hold 2 * (blockDim.x / 64) doubles.
+__device__ __forceinline__ double block_ksum(KSum k, double* scratch) {
+    k = wave_ksum(k);
+    const int lane = threadIdx.x % kWave;
+    const int wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    if (lane == 0) {
+        scratch[2 * wid] = k.s;
+        scratch[2 * wid + 1] = k.c;
+    }
+    __syncthreads();
+    KSum t{0.0, 0.0};
+    for (int w = 0; w < nw; ++w) {
+        KSum o{scratch[2 * w], scratch[2 * w + 1]};
+        ksum_merge(t, o);
+    }
+    __syncthreads();
+    return t.s + t.c;
+}
+
+}  // namespace vp

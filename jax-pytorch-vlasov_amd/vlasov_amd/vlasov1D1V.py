@@ -1,0 +1,392 @@
+"""1D1V Vlasov-Poisson solver: drop-in for the reference PyTorch module.
+
+Mirrors the public interface of
+``simulations/vlasov1d_1v/pytorch/src/vlasov1D1V.py`` (cited as P1 below):
+the same ``Grid`` / ``Field`` / ``Variables`` / ``SimulationState``
+dataclasses, the same kernel-level functions (``advect_1d_vectorized``,
+``solve_poisson``, ``solve_poisson_real``, ``vlasov_step``) with the same
+positional signatures, and the same ``Vlasov`` driver.  Every computation
+runs in the hand-written HIP kernels of ``libvlasov_amd.so``; tensors must be
+on a ROCm GPU (``device='cuda'``), there is no CPU path.
+"""
+from __future__ import annotations
+
+import math
+import pathlib
+import platform
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, require_gpu, stream_handle
+
+
+# ---------------------------------------------------------------------------
+# Device helpers (P1:19-74)
+# ---------------------------------------------------------------------------
+
+def get_device_label(device: torch.device) -> str:
+    if device.type == "cuda":
+        idx = device.index if device.index is not None else torch.cuda.current_device()
+        return f"{torch.cuda.get_device_name(idx)} (cuda:{idx})"
+    return f"{platform.processor() or platform.machine()} (cpu)"
+
+
+def synchronize(device: torch.device):
+    if device.type == "cuda":
+        torch.cuda.synchronize()
+
+
+def reset_peak_memory(device: torch.device):
+    if device.type == "cuda":
+        torch.cuda.reset_peak_memory_stats(device)
+
+
+def get_memory_stats(device: torch.device):
+    synchronize(device)
+    if device.type == "cuda":
+        return {
+            "allocated": torch.cuda.memory_allocated(device),
+            "reserved": torch.cuda.memory_reserved(device),
+            "peak_allocated": torch.cuda.max_memory_allocated(device),
+            "peak_reserved": torch.cuda.max_memory_reserved(device),
+        }
+    return None
+
+
+def format_memory(num_bytes: int) -> str:
+    return f"{num_bytes / (1024 ** 2):.2f} MiB"
+
+
+def default_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("vlasov_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda")
+
+
+# ---------------------------------------------------------------------------
+# Data structures (P1:76-160)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Grid:
+    Nx: int
+    Nv: int
+    L: float
+    V_max: float
+    dx: float
+    dv: float
+    x: torch.Tensor
+    v: torch.Tensor
+    inv_kx: torch.Tensor
+    inv_kx_sq: torch.Tensor
+    x_indices: torch.Tensor
+    v_indices: torch.Tensor
+    x_batch_offsets: torch.Tensor
+    v_batch_offsets: torch.Tensor
+    stencil_offsets: torch.Tensor
+
+    @classmethod
+    def create(cls, Nx, Nv, L, V_max, device="cuda", dtype=torch.float64):
+        """Grid of P1:94-122 (same torch expressions, so identical arrays)."""
+        dx = L / Nx
+        dv = 2 * V_max / (Nv - 1)
+        x = torch.linspace(0, L, Nx + 1, device=device, dtype=dtype)[:-1]
+        v = torch.linspace(-V_max, V_max, Nv, device=device, dtype=dtype)
+        kx = 2 * math.pi * torch.fft.rfftfreq(Nx, d=dx, device=device, dtype=dtype)
+        inv_kx = torch.where(kx == 0, torch.tensor(0.0, device=device, dtype=dtype), 1.0 / kx)
+        inv_kx_sq = inv_kx * inv_kx
+        index_dtype = torch.int64
+        x_indices = torch.arange(Nx, device=device, dtype=dtype)
+        v_indices = torch.arange(Nv, device=device, dtype=dtype)
+        x_batch_offsets = (torch.arange(Nv, device=device, dtype=index_dtype) * Nx).view(Nv, 1, 1)
+        v_batch_offsets = (torch.arange(Nx, device=device, dtype=index_dtype) * Nv).view(Nx, 1, 1)
+        stencil_offsets = torch.arange(-2, 4, device=device, dtype=index_dtype)
+        return Grid(Nx, Nv, L, V_max, dx, dv, x, v, inv_kx, inv_kx_sq,
+                    x_indices, v_indices, x_batch_offsets, v_batch_offsets, stencil_offsets)
+
+    def to(self, device) -> "Grid":
+        for name in ("x", "v", "inv_kx", "inv_kx_sq", "x_indices", "v_indices",
+                     "x_batch_offsets", "v_batch_offsets", "stencil_offsets"):
+            setattr(self, name, getattr(self, name).to(device))
+        return self
+
+
+@dataclass
+class Field:
+    """Field data (rho, phi)."""
+    rho: torch.Tensor
+    phi: torch.Tensor
+
+
+@dataclass
+class Variables:
+    """Distribution function f and the equilibrium f0."""
+    fn: torch.Tensor
+    f0: torch.Tensor
+
+
+@dataclass
+class SimulationState:
+    field: Field
+    variables: Variables
+    t: float
+    step: int
+
+
+# ---------------------------------------------------------------------------
+# Kernels
+# ---------------------------------------------------------------------------
+
+def _f64(t: torch.Tensor) -> torch.Tensor:
+    if t.dtype != torch.float64:
+        raise TypeError(f"the float64 path needs float64 tensors, got {t.dtype}")
+    return t
+
+
+def advect_rows(f_batch: torch.Tensor, velocities: torch.Tensor, dt: float, dx: float,
+                periodic: bool = True) -> torch.Tensor:
+    """Sweep every row of f_batch [B, N] (the advected axis is the last one).
+
+    ``velocities`` is [B] (one velocity per row) or [B, N] (per point), as in
+    advect_1d_vectorized (P1:212-256).  Returns a new [B, N] tensor.
+    """
+    dev = require_gpu(f_batch, velocities)
+    B, N = f_batch.shape
+    vel = velocities.to(torch.float64)
+    if vel.ndim == 1:
+        vel = vel.expand(B)
+        vs = (0, 0, vel.stride(0), 0)
+    else:
+        vel = vel.expand(B, N)
+        vs = (0, 0, vel.stride(0), vel.stride(1))
+    return _advect(f_batch, (1, 1, B, N), 3, vel, vs, dt, dx, periodic, dev)
+
+
+def _advect(f, shape, axis, vel, vs, dt, dx, periodic, dev, out=None):
+    f_c = f.contiguous()
+    if out is None:
+        out = torch.empty_like(f_c)
+    if f_c.dtype == torch.float64:
+        fn = lib().vp_advect_f64
+    elif f_c.dtype == torch.float32:
+        fn = lib().vp_advect_f32
+    else:
+        raise TypeError(f"unsupported dtype {f_c.dtype}")
+    rc = fn(f_c.data_ptr(), out.data_ptr(), _lib.i64x4(shape), int(axis), vel.data_ptr(),
+            _lib.i64x4(vs), float(dt), float(dx), int(bool(periodic)), stream_handle(dev))
+    check(rc, "vp_advect")
+    return out
+
+
+def advect_1d_vectorized(f_batch, velocities, dt, point_indices, batch_offsets, stencil_offsets, dx,
+                         periodic=True):
+    """Drop-in for advect_1d_vectorized (P1:212-256).
+
+    ``point_indices``, ``batch_offsets`` and ``stencil_offsets`` are accepted
+    for signature compatibility; the kernel derives the same indices itself
+    (arange(N), row offsets, taps -2..3).
+    """
+    return advect_rows(f_batch, velocities, dt, dx, periodic=periodic)
+
+
+def lagrange5_weights(alpha):
+    """[6, ...] stacked power-form weights (P1:191-210); element-wise torch ops."""
+    a = alpha
+    a2 = a * a
+    a3 = a2 * a
+    a4 = a3 * a
+    a5 = a4 * a
+    w0 = (a / 20.0) - (a2 / 24.0) - (a3 / 24.0) + (a4 / 24.0) - (a5 / 120.0)
+    w1 = -(a / 2.0) + (2.0 * a2 / 3.0) - (a3 / 24.0) - (a4 / 6.0) + (a5 / 24.0)
+    w2 = 1.0 - (a / 3.0) - (5.0 * a2 / 4.0) + (5.0 * a3 / 12.0) + (a4 / 4.0) - (a5 / 12.0)
+    w3 = a + (2.0 * a2 / 3.0) - (7.0 * a3 / 12.0) - (a4 / 6.0) + (a5 / 12.0)
+    w4 = -(a / 4.0) - (a2 / 24.0) + (7.0 * a3 / 24.0) + (a4 / 24.0) - (a5 / 24.0)
+    w5 = (a / 30.0) - (a3 / 24.0) + (a5 / 120.0)
+    return torch.stack([w0, w1, w2, w3, w4, w5], dim=0)
+
+
+def lagrange5_interp_sum(alpha, gather_vals):
+    """sum_k w_k(alpha) gather_vals[..., k] (P1:163-189); element-wise torch ops."""
+    w = lagrange5_weights(alpha)
+    return (w[0] * gather_vals[..., 0] + w[1] * gather_vals[..., 1] + w[2] * gather_vals[..., 2]
+            + w[3] * gather_vals[..., 3] + w[4] * gather_vals[..., 4] + w[5] * gather_vals[..., 5])
+
+
+def compute_rho(f, dv):
+    """rho = sum_v f * dv - 1 (P1:559-561), compensated float64 sum."""
+    dev = require_gpu(f)
+    f_c = f.contiguous()
+    Nx, Nv = f_c.shape
+    rho = torch.empty(Nx, dtype=torch.float64, device=dev)
+    fn = lib().vp_row_sum_f64 if f_c.dtype == torch.float64 else lib().vp_row_sum_f32
+    check(fn(f_c.data_ptr(), Nx, Nv, float(dv), 1.0, 1.0, rho.data_ptr(), stream_handle(dev)),
+          "vp_row_sum")
+    return rho
+
+
+def poisson_from_rho(rho, inv_kx, inv_kx_sq):
+    """E and phi from rho (P1:551-572)."""
+    dev = require_gpu(rho, inv_kx, inv_kx_sq)
+    Nx = rho.shape[0]
+    E = torch.empty(Nx, dtype=torch.float64, device=dev)
+    phi = torch.empty(Nx, dtype=torch.float64, device=dev)
+    nbytes = lib().vp_poisson_1d_workspace(Nx)
+    ws = _lib.Workspace.get(dev, nbytes, "poisson1d")
+    rc = lib().vp_poisson_1d_f64(rho.contiguous().data_ptr(), Nx, _f64(inv_kx).contiguous().data_ptr(),
+                                 _f64(inv_kx_sq).contiguous().data_ptr(), E.data_ptr(), phi.data_ptr(),
+                                 ws.data_ptr(), ws.numel(), stream_handle(dev))
+    check(rc, "vp_poisson_1d")
+    return E, phi
+
+
+def solve_poisson_real(f, inv_kx, inv_kx_sq, dv):
+    """Drop-in for solve_poisson_real (P1:538-572).  Returns (E, rho, phi)."""
+    rho = compute_rho(f, dv)
+    E, phi = poisson_from_rho(rho, inv_kx, inv_kx_sq)
+    return E, rho, phi
+
+
+def solve_poisson(f, inv_kx, dv):
+    """Drop-in for solve_poisson (P1:514-536): same results as the real form."""
+    return solve_poisson_real(f, inv_kx, inv_kx * inv_kx, dv)
+
+
+def vlasov_step(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, x_indices, v_indices, x_batch_offsets,
+                v_batch_offsets, stencil_offsets, grid_dx, grid_dv, grid_V_max, physics_mode):
+    """Drop-in for vlasov_step (P1:574-622): X(dt/2), Poisson, V(dt), X(dt/2).
+
+    Functional: returns new (f, rho, phi); f is not modified.
+    """
+    f, rho, phi, _ = step_with_field(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, grid_dx, grid_dv,
+                                     physics_mode)
+    return f, rho, phi
+
+
+def step_with_field(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, grid_dx, grid_dv, physics_mode):
+    """One vlasov_step that also returns the mid-step field E: (f, rho, phi, E)."""
+    dev = require_gpu(f, grid_v)
+    f_c = _f64(f).contiguous()
+    Nx, Nv = f_c.shape
+    out = torch.empty_like(f_c)
+    E = torch.empty(Nx, dtype=torch.float64, device=dev)
+    rho = torch.empty(Nx, dtype=torch.float64, device=dev)
+    phi = torch.empty(Nx, dtype=torch.float64, device=dev)
+    nbytes = lib().vp_step_1d1v_workspace(Nx, Nv)
+    ws = _lib.Workspace.get(dev, nbytes, "step1d")
+    pm = bool(physics_mode)
+    rc = lib().vp_step_1d1v_f64(
+        f_c.data_ptr(), out.data_ptr(), Nx, Nv, float(dt), _f64(grid_v).contiguous().data_ptr(),
+        ptr(grid_inv_kx.contiguous()) if pm else None, ptr(grid_inv_kx_sq.contiguous()) if pm else None,
+        float(grid_dx), float(grid_dv), int(pm), E.data_ptr(), rho.data_ptr(), phi.data_ptr(),
+        ws.data_ptr(), ws.numel(), stream_handle(dev))
+    check(rc, "vp_step_1d1v")
+    return out, rho, phi, E
+
+
+# ---------------------------------------------------------------------------
+# Driver (P1:757-994)
+# ---------------------------------------------------------------------------
+
+class Vlasov:
+    dt: float = 0.05
+    diag_steps: int = 10
+    out_dir: str = "data"
+    physics_mode: bool = False
+    epsilon: float = 0.001
+    dtype: torch.dtype = torch.float64
+
+    def __init__(self, Nx: int = 128, Nv: int = 128, L: float = 2.0 * math.pi, V_max: float = 6.0,
+                 nbiter: int = 10, diag_steps: int = 10, dt: float = 0.05, out_dir: str = "data",
+                 physics_mode: bool = False, epsilon: float = 0.001, dtype: str = "float64",
+                 solver: int = 0, device=None) -> None:
+        """Same parameters as the reference constructor (P1:770-876).
+
+        ``solver`` keeps the reference ids for log compatibility; every id
+        runs the HIP step, which follows solver 0's arithmetic
+        (vlasov_step).  Only float64 is supported by this class.
+        """
+        self.output_log = ""
+        if dtype != "float64":
+            raise ValueError("vlasov_amd.vlasov1D1V.Vlasov runs float64 (dtype='float64')")
+        self.dtype = torch.float64
+        self.device = torch.device(device) if device is not None else default_device()
+        self.grid = Grid.create(Nx, Nv, L, V_max, device=self.device, dtype=self.dtype)
+        self.nbiter = nbiter
+        self.diag_steps = diag_steps
+        self.dt = dt
+        self.out_dir = out_dir
+        self.physics_mode = physics_mode
+        self.epsilon = epsilon
+        solver_names = {0: "vlasov_step", 1: "vlasov_step_matrix_diag", 2: "vlasov_step_gather_free"}
+        self.solver = solver
+        self.solver_type = solver_names[solver]
+        self._step_fn = vlasov_step
+        self.energy_history: list = []
+        pathlib.Path(self.out_dir).mkdir(parents=True, exist_ok=True)
+        num_steps = self.nbiter // self.diag_steps
+        message = f"PyTorch version: {torch.__version__}\n"
+        message += f"Starting simulation on {self.device} with {self.grid.x.dtype} precision "
+        message += "with Poisson solver" if self.physics_mode else "without Poisson solver"
+        message += f" and solver type {self.solver_type}."
+        self._append_log(message)
+        self._append_log(f"Grid: {self.grid.Nx}x{self.grid.Nv}, Steps: {self.nbiter}, "
+                         f"chunks: {num_steps}, diag_freq: {self.diag_steps}")
+
+    def _append_log(self, message: str) -> None:
+        self.output_log += f"{message}\n"
+
+    def initialize(self) -> SimulationState:
+        """Initial state of P1:881-911 (Landau damping or Gaussian pulse)."""
+        x_2d = self.grid.x.unsqueeze(1)
+        v_2d = self.grid.v.unsqueeze(0)
+        if self.physics_mode:
+            kx = 2.0 * math.pi / self.grid.L
+            maxwellian = (1.0 / math.sqrt(2.0 * math.pi)) * torch.exp(-0.5 * v_2d ** 2)
+            perturbation = 1.0 + self.epsilon * torch.cos(kx * x_2d)
+            f = perturbation * maxwellian
+            f0 = maxwellian.squeeze(0)
+        else:
+            x0, v0 = self.grid.L / 2.0, 0.0
+            sigma_x, sigma_v = self.grid.L / 10.0, self.grid.V_max / 4.0
+            f = torch.exp(-0.5 * ((x_2d - x0) ** 2 / sigma_x ** 2 + (v_2d - v0) ** 2 / sigma_v ** 2))
+            f0 = torch.zeros_like(self.grid.v, dtype=self.dtype)
+        E, rho, phi = solve_poisson_real(f, self.grid.inv_kx, self.grid.inv_kx_sq, self.grid.dv)
+        state = SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=f0), t=0.0, step=0)
+        self.save_diag(state)
+        return state
+
+    def save_diag(self, state: SimulationState) -> None:
+        """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray)."""
+        filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
+        np.savez(filename, f=state.variables.fn.detach().cpu().numpy(),
+                 f0=state.variables.f0.detach().cpu().numpy(),
+                 rho=state.field.rho.detach().cpu().numpy(), phi=state.field.phi.detach().cpu().numpy(),
+                 time=np.array([float(state.t)]), x=self.grid.x.cpu().numpy(), v=self.grid.v.cpu().numpy(),
+                 step=int(state.step), dt=self.dt, physics_mode=int(self.physics_mode))
+
+    def run(self, state: SimulationState) -> SimulationState:
+        """Time loop of P1:943-994."""
+        g = self.grid
+        diag_counter = 0
+        rho, phi = state.field.rho, state.field.phi
+        f = state.variables.fn
+        synchronize(self.device)
+        for _ in range(1, self.nbiter + 1):
+            f, rho, phi = self._step_fn(f, self.dt, g.v, g.inv_kx, g.inv_kx_sq, g.x_indices, g.v_indices,
+                                        g.x_batch_offsets, g.v_batch_offsets, g.stencil_offsets, g.dx, g.dv,
+                                        g.V_max, self.physics_mode)
+            state.t += self.dt
+            state.step += 1
+            diag_counter += 1
+            if diag_counter >= self.diag_steps:
+                synchronize(self.device)
+                self.save_diag(SimulationState(field=Field(rho, phi),
+                                               variables=Variables(fn=f, f0=state.variables.f0),
+                                               t=state.t, step=state.step))
+                diag_counter = 0
+        synchronize(self.device)
+        return SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=state.variables.f0),
+                               t=state.t, step=state.step)

@@ -1,0 +1,369 @@
+"""2D2V Vlasov-Poisson solver: drop-in for the reference PyTorch module.
+
+Mirrors the public interface of
+``simulations/vlasov2d_2v/pytorch/src/vlasov2D2V.py`` (cited as P2 below):
+``Grid`` / ``Field`` / ``Variables`` / ``SimulationState``, the kernel-level
+functions ``advect_1d_vectorized``, ``advect_1d``, ``compute_rho``,
+``compute_poisson(_real)``, ``solve_poisson(_real)``, the step
+``vlasov_step_impl`` (same 22 positional arguments) and the ``Vlasov``
+driver.  Everything runs in the HIP kernels of ``libvlasov_amd.so`` on a
+ROCm GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import math
+import pathlib
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, require_gpu, stream_handle
+from .vlasov1D1V import (default_device, format_memory, get_device_label, get_memory_stats,  # noqa: F401
+                         lagrange5_interp_sum, lagrange5_weights, reset_peak_memory, synchronize)
+
+
+@dataclass
+class Grid:
+    """Grid of the 2D2V solver (P2:88-189)."""
+    Nx: int
+    Ny: int
+    Nvx: int
+    Nvy: int
+    Lx: float
+    Ly: float
+    Vx_max: float
+    Vy_max: float
+    dx: float
+    dy: float
+    dvx: float
+    dvy: float
+    x: torch.Tensor
+    y: torch.Tensor
+    vx: torch.Tensor
+    vy: torch.Tensor
+    kx: torch.Tensor
+    ky: torch.Tensor
+    inv_ksq: torch.Tensor
+    x_indices: torch.Tensor
+    y_indices: torch.Tensor
+    vx_indices: torch.Tensor
+    vy_indices: torch.Tensor
+    stencil_offsets: torch.Tensor
+
+    @classmethod
+    def create(cls, Nx, Ny, Nvx, Nvy, Lx, Ly, Vx_max, Vy_max, device="cuda", dtype=torch.float64):
+        """Same torch expressions as P2:146-173, so identical arrays."""
+        dx, dy = Lx / Nx, Ly / Ny
+        dvx, dvy = 2 * Vx_max / (Nvx - 1), 2 * Vy_max / (Nvy - 1)
+        x = torch.linspace(0, Lx, Nx + 1, device=device, dtype=dtype)[:-1]
+        y = torch.linspace(0, Ly, Ny + 1, device=device, dtype=dtype)[:-1]
+        vx = torch.linspace(-Vx_max, Vx_max, Nvx, device=device, dtype=dtype)
+        vy = torch.linspace(-Vy_max, Vy_max, Nvy, device=device, dtype=dtype)
+        kx = 2 * math.pi * torch.fft.fftfreq(Nx, d=dx, device=device, dtype=dtype).unsqueeze(1)
+        ky = 2 * math.pi * torch.fft.rfftfreq(Ny, d=dy, device=device, dtype=dtype).unsqueeze(0)
+        k_sq = kx ** 2 + ky ** 2
+        inv_ksq = torch.where(k_sq == 0, torch.tensor(0.0, device=device, dtype=dtype), 1.0 / k_sq)
+        x_indices = torch.arange(Nx, device=device, dtype=dtype)
+        y_indices = torch.arange(Ny, device=device, dtype=dtype)
+        vx_indices = torch.arange(Nvx, device=device, dtype=dtype)
+        vy_indices = torch.arange(Nvy, device=device, dtype=dtype)
+        stencil_offsets = torch.arange(-2, 4, device=device, dtype=torch.int64)
+        return cls(Nx, Ny, Nvx, Nvy, Lx, Ly, Vx_max, Vy_max, dx, dy, dvx, dvy, x, y, vx, vy, kx, ky,
+                   inv_ksq, x_indices, y_indices, vx_indices, vy_indices, stencil_offsets)
+
+    def to(self, device) -> "Grid":
+        for name in ("x", "y", "vx", "vy", "kx", "ky", "inv_ksq", "x_indices", "y_indices",
+                     "vx_indices", "vy_indices", "stencil_offsets"):
+            setattr(self, name, getattr(self, name).to(device))
+        return self
+
+
+@dataclass
+class Field:
+    """Field data (rho, phi)."""
+    rho: torch.Tensor
+    phi: torch.Tensor
+
+
+@dataclass
+class Variables:
+    """Distribution function f and f0."""
+    fn: torch.Tensor
+    f0: torch.Tensor
+
+
+@dataclass
+class SimulationState:
+    field: Field
+    variables: Variables
+    t: float
+    step: int
+
+
+def get_lagrange5_weights(alpha):
+    """[..., 6] weights in the generic product form (P2:210-235); torch element-wise."""
+    weights = torch.empty((*alpha.shape, 6), device=alpha.device, dtype=alpha.dtype)
+    nodes = torch.arange(-2, 4, device=alpha.device, dtype=alpha.dtype)
+    for j in range(6):
+        w = torch.ones_like(alpha)
+        for m in range(6):
+            if m != j:
+                w = w * (alpha - nodes[m]) / (nodes[j] - nodes[m])
+        weights[..., j] = w
+    return weights
+
+
+# ---------------------------------------------------------------------------
+# Sweeps
+# ---------------------------------------------------------------------------
+
+def _advect4(f, vel_f_strides, vel, axis, dt, dx, periodic, out=None):
+    dev = require_gpu(f, vel)
+    f_c = f.contiguous()
+    if out is None:
+        out = torch.empty_like(f_c)
+    fn = lib().vp_advect_f64 if f_c.dtype == torch.float64 else lib().vp_advect_f32
+    if f_c.dtype not in (torch.float64, torch.float32):
+        raise TypeError(f"unsupported dtype {f_c.dtype}")
+    rc = fn(f_c.data_ptr(), out.data_ptr(), _lib.i64x4(f_c.shape), int(axis), vel.data_ptr(),
+            _lib.i64x4(vel_f_strides), float(dt), float(dx), int(bool(periodic)), stream_handle(dev))
+    check(rc, "vp_advect")
+    return out
+
+
+def advect_1d(f_data, velocities, axis, dt, dx, periodic=True, point_indices=None, stencil_offsets=None):
+    """Drop-in for advect_1d (P2:356-413): sweep of f(x, y, vx, vy) along ``axis``.
+
+    ``velocities`` broadcasts to the batch shape (all dims except ``axis``),
+    exactly as in the reference; the sweep runs in place of the
+    permute/reshape/gather, reading each line once through LDS.
+    """
+    if f_data.ndim != 4:
+        raise ValueError("advect_1d expects a 4D f(x, y, vx, vy)")
+    axis = int(axis) % 4
+    batch_shape = tuple(s for i, s in enumerate(f_data.shape) if i != axis)
+    vel = velocities.to(device=f_data.device, dtype=torch.float64).expand(batch_shape)
+    strides = list(vel.stride())
+    strides.insert(axis, 0)
+    return _advect4(f_data, strides, vel, axis, dt, dx, periodic)
+
+
+def advect_1d_vectorized(f_batch, velocities, dt, dx, point_indices=None, stencil_offsets=None, periodic=True):
+    """Drop-in for advect_1d_vectorized (P2:287-353) on rows [B, N]."""
+    B, N = f_batch.shape
+    vel = velocities.to(device=f_batch.device, dtype=torch.float64)
+    if vel.ndim == 1:
+        vel = vel.expand(B)
+        strides = (0, 0, vel.stride(0), 0)
+    else:
+        vel = vel.expand(B, N)
+        strides = (0, 0, vel.stride(0), vel.stride(1))
+    return _advect4(f_batch.reshape(1, 1, B, N), strides, vel, 3, dt, dx, periodic).reshape(B, N)
+
+
+# ---------------------------------------------------------------------------
+# Field solve
+# ---------------------------------------------------------------------------
+
+def compute_rho(f_data, dvx, dvy):
+    """Drop-in for compute_rho (P2:416-431), compensated float64 sum."""
+    dev = require_gpu(f_data)
+    f_c = f_data.contiguous()
+    Nx, Ny, Nvx, Nvy = f_c.shape
+    rho = torch.empty((Nx, Ny), dtype=torch.float64, device=dev)
+    fn = lib().vp_row_sum_f64 if f_c.dtype == torch.float64 else lib().vp_row_sum_f32
+    check(fn(f_c.data_ptr(), Nx * Ny, Nvx * Nvy, float(dvx), float(dvy), 0.0, rho.data_ptr(),
+             stream_handle(dev)), "vp_row_sum")
+    return rho
+
+
+def compute_poisson_real(rho, kx, ky, inv_ksq):
+    """Drop-in for compute_poisson_real (P2:488-517).  Returns (Ex, Ey, delta_rho, phi)."""
+    dev = require_gpu(rho, kx, ky, inv_ksq)
+    Nx, Ny = rho.shape
+    outs = [torch.empty((Nx, Ny), dtype=torch.float64, device=dev) for _ in range(4)]
+    Ex, Ey, drho, phi = outs
+    ws = _lib.Workspace.get(dev, lib().vp_poisson_2d_workspace(Nx, Ny), "poisson2d")
+    f64 = torch.float64
+    rc = lib().vp_poisson_2d_f64(rho.to(f64).contiguous().data_ptr(), Nx, Ny,
+                                 kx.to(f64).contiguous().data_ptr(), ky.to(f64).contiguous().data_ptr(),
+                                 inv_ksq.to(f64).contiguous().data_ptr(), Ex.data_ptr(), Ey.data_ptr(),
+                                 drho.data_ptr(), phi.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle(dev))
+    check(rc, "vp_poisson_2d")
+    return Ex, Ey, drho, phi
+
+
+def compute_poisson(rho, kx, ky, inv_ksq):
+    """Drop-in for compute_poisson (P2:434-465): same results as the real form."""
+    return compute_poisson_real(rho, kx, ky, inv_ksq)
+
+
+def solve_poisson_real(f_data, kx, ky, inv_ksq, dvx, dvy):
+    """Drop-in for solve_poisson_real (P2:520-523)."""
+    return compute_poisson_real(compute_rho(f_data, dvx, dvy), kx, ky, inv_ksq)
+
+
+def solve_poisson(f_data, kx, ky, inv_ksq, dvx, dvy):
+    """Drop-in for solve_poisson (P2:468-485)."""
+    return solve_poisson_real(f_data, kx, ky, inv_ksq, dvx, dvy)
+
+
+# ---------------------------------------------------------------------------
+# Step
+# ---------------------------------------------------------------------------
+
+def step_with_field(f, dt, grid_vx, grid_vy, grid_kx, grid_ky, grid_inv_ksq, grid_dx, grid_dy, grid_dvx,
+                    grid_dvy, physics_mode, out=None):
+    """One Strang step; returns (f, delta_rho, phi, Ex, Ey)."""
+    dev = require_gpu(f, grid_vx, grid_vy)
+    if f.dtype != torch.float64:
+        raise TypeError("step_with_field runs float64")
+    f_c = f.contiguous()
+    Nx, Ny = f_c.shape[0], f_c.shape[1]
+    if out is None:
+        out = torch.empty_like(f_c)
+    fields = torch.empty((4, Nx, Ny), dtype=torch.float64, device=dev)
+    Ex, Ey, drho, phi = fields[0], fields[1], fields[2], fields[3]
+    shape = _lib.i64x4(f_c.shape)
+    ws = _lib.Workspace.get(dev, lib().vp_step_2d2v_workspace(shape), "step2d")
+    pm = bool(physics_mode)
+    f64 = torch.float64
+    kx = grid_kx.to(f64).contiguous() if pm else None
+    ky = grid_ky.to(f64).contiguous() if pm else None
+    iks = grid_inv_ksq.to(f64).contiguous() if pm else None
+    rc = lib().vp_step_2d2v_f64(
+        f_c.data_ptr(), out.data_ptr(), shape, float(dt), grid_vx.to(f64).contiguous().data_ptr(),
+        grid_vy.to(f64).contiguous().data_ptr(), _lib.ptr(kx), _lib.ptr(ky), _lib.ptr(iks), float(grid_dx),
+        float(grid_dy), float(grid_dvx), float(grid_dvy), int(pm), Ex.data_ptr(), Ey.data_ptr(),
+        drho.data_ptr(), phi.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle(dev))
+    check(rc, "vp_step_2d2v")
+    return out, drho, phi, Ex, Ey
+
+
+def vlasov_step_impl(f, f0, dt, grid_x, grid_y, grid_vx, grid_vy, grid_kx, grid_ky, grid_inv_ksq,
+                     grid_x_indices, grid_y_indices, grid_vx_indices, grid_vy_indices, grid_stencil_offsets,
+                     grid_dx, grid_dy, grid_dvx, grid_dvy, grid_Nx, grid_Ny, physics_mode):
+    """Drop-in for vlasov_step_impl (P2:943-998).  Returns (f, rho, phi)."""
+    f, rho, phi, _, _ = step_with_field(f, dt, grid_vx, grid_vy, grid_kx, grid_ky, grid_inv_ksq, grid_dx,
+                                        grid_dy, grid_dvx, grid_dvy, physics_mode)
+    return f, rho, phi
+
+
+# ---------------------------------------------------------------------------
+# Driver (P2:1223-1435)
+# ---------------------------------------------------------------------------
+
+class Vlasov:
+    dt: float = 0.05
+    diag_steps: int = 10
+    out_dir: str = "data"
+    physics_mode: bool = False
+    epsilon: float = 0.001
+    dtype: torch.dtype = torch.float64
+
+    def __init__(self, Nx=32, Ny=32, Nvx=32, Nvy=32, Lx=2.0 * math.pi, Ly=2.0 * math.pi, Vx_max=6.0,
+                 Vy_max=6.0, nbiter=10, diag_steps=10, dt=0.05, out_dir="data", physics_mode=False,
+                 epsilon=0.001, dtype: str = "float64", solver: int = 0, device=None):
+        """Same parameters as the reference constructor (P2:1236-1314).
+
+        ``solver`` keeps the reference ids for the log; every id runs the HIP
+        step, which follows solver 0's arithmetic (vlasov_step_impl).
+        """
+        self.output_log = ""
+        if dtype != "float64":
+            raise ValueError("vlasov_amd.vlasov2D2V.Vlasov runs float64 (dtype='float64')")
+        self.dtype = torch.float64
+        self.device = torch.device(device) if device is not None else default_device()
+        self.grid = Grid.create(Nx, Ny, Nvx, Nvy, Lx, Ly, Vx_max, Vy_max, device=self.device, dtype=self.dtype)
+        self.nbiter = nbiter
+        self.diag_steps = diag_steps
+        self.dt = dt
+        self.out_dir = out_dir
+        self.physics_mode = physics_mode
+        self.epsilon = epsilon
+        solver_names = {0: "vlasov_step_impl", 1: "vlasov_step_matrix_diag", 2: "vlasov_step_gather_free"}
+        self.solver = solver
+        self.solver_type = solver_names[solver]
+        self._step_fn = vlasov_step_impl
+        pathlib.Path(self.out_dir).mkdir(parents=True, exist_ok=True)
+        num_steps = self.nbiter // self.diag_steps
+        message = f"PyTorch version: {torch.__version__}\n"
+        message += f"Starting simulation on {self.device} with {self.grid.x.dtype} precision "
+        message += "with Poisson solver" if self.physics_mode else "without Poisson solver"
+        message += f" and solver type {self.solver_type}."
+        self._append_log(message)
+        self._append_log(f"Grid: {Nx}x{Ny}x{Nvx}x{Nvy}, Steps: {self.nbiter}, chunks: {num_steps}, "
+                         f"diag_freq: {self.diag_steps}")
+
+    def _append_log(self, message: str) -> None:
+        self.output_log += f"{message}\n"
+
+    def initialize(self) -> SimulationState:
+        """Initial state of P2:1319-1355."""
+        g = self.grid
+        x_mesh = g.x[:, None, None, None]
+        y_mesh = g.y[None, :, None, None]
+        vx_mesh = g.vx[:, None]
+        vy_mesh = g.vy[None, :]
+        if self.physics_mode:
+            kx_mode = 1 * 2.0 * math.pi / g.Lx
+            ky_mode = 1 * 2.0 * math.pi / g.Ly
+            energy = 0.5 * (vx_mesh ** 2 + vy_mesh ** 2)
+            f0 = (1.0 / math.sqrt(2.0 * math.pi)) * torch.exp(-energy)
+            f_data = (1.0 + self.epsilon * torch.cos(kx_mode * x_mesh) * torch.cos(ky_mode * y_mesh)) \
+                * f0[None, None, :, :]
+        else:
+            x0, y0 = g.Lx / 2.0, g.Ly / 2.0
+            sigma_x = g.Lx / 10.0
+            sigma_v = g.Vx_max / 4.0
+            dx2 = (x_mesh - x0) ** 2 / sigma_x ** 2
+            dy2 = (y_mesh - y0) ** 2 / sigma_x ** 2
+            vx2 = (vx_mesh[None, None, :, :] - 0.0) ** 2 / sigma_v ** 2
+            vy2 = (vy_mesh[None, None, :, :] - 0.0) ** 2 / sigma_v ** 2
+            f_data = torch.exp(-0.5 * (dx2 + dy2 + vx2 + vy2))
+            f0 = torch.zeros((g.Nvx, g.Nvy), device=self.device, dtype=self.dtype)
+        *_, rho, phi = solve_poisson_real(f_data, g.kx, g.ky, g.inv_ksq, g.dvx, g.dvy)
+        state = SimulationState(field=Field(rho, phi), variables=Variables(fn=f_data, f0=f0), t=0.0, step=0)
+        self.save_diag(state)
+        return state
+
+    def save_diag(self, state: SimulationState) -> None:
+        """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray)."""
+        g = self.grid
+        filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
+        np.savez(filename, f=state.variables.fn.detach().cpu().numpy(), f0=state.variables.f0.detach().cpu().numpy(),
+                 rho=state.field.rho.detach().cpu().numpy(), phi=state.field.phi.detach().cpu().numpy(),
+                 time=np.array([float(state.t)]), x=g.x.cpu().numpy(), y=g.y.cpu().numpy(),
+                 vx=g.vx.cpu().numpy(), vy=g.vy.cpu().numpy(), step=int(state.step), dt=self.dt,
+                 Lx=g.Lx, Ly=g.Ly, Vx_max=g.Vx_max, Vy_max=g.Vy_max, physics_mode=int(self.physics_mode))
+
+    def run(self, state: SimulationState) -> SimulationState:
+        """Time loop of P2:1392-1435."""
+        g = self.grid
+        diag_counter = 0
+        rho, phi = state.field.rho, state.field.phi
+        f = state.variables.fn
+        synchronize(self.device)
+        for _ in range(1, self.nbiter + 1):
+            f, rho, phi = self._step_fn(f, state.variables.f0, self.dt, g.x, g.y, g.vx, g.vy, g.kx, g.ky,
+                                        g.inv_ksq, g.x_indices, g.y_indices, g.vx_indices, g.vy_indices,
+                                        g.stencil_offsets, g.dx, g.dy, g.dvx, g.dvy, g.Nx, g.Ny,
+                                        self.physics_mode)
+            state.t += self.dt
+            state.step += 1
+            diag_counter += 1
+            if diag_counter >= self.diag_steps:
+                synchronize(self.device)
+                self.save_diag(SimulationState(field=Field(rho, phi),
+                                               variables=Variables(fn=f, f0=state.variables.f0),
+                                               t=state.t, step=state.step))
+                diag_counter = 0
+        synchronize(self.device)
+        return SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=state.variables.f0),
+                               t=state.t, step=state.step)
+
+
+__all__ = [name for name in dir() if not name.startswith("_")]
+_ = format_memory
