@@ -112,13 +112,74 @@ int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, 
                      double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
 
 /* vlasov_step_impl (P2:943-998): X, Y (dt/2), Poisson, Vx, Vy (dt), Y, X (dt/2).
- * Ex, Ey, delta_rho, phi: [nx][ny] outputs; zero when physics_mode == 0. */
+ * Ex, Ey, delta_rho, phi: [nx][ny] outputs; zero when physics_mode == 0.
+ * For nx, ny, nvx, nvy in {64, 128} the step runs the fused in-place kernels
+ * (X drift, Y drift + rho, Vx+Vy kick per (x,y) plane); other shapes run the
+ * generic sweeps.  Both give identical f. */
 size_t vp_step_2d2v_workspace(const int64_t shape[4]);
 int vp_step_2d2v_f64(const double* f_in, double* f_out, const int64_t shape[4], double dt,
                      const double* vx, const double* vy, const double* kx, const double* ky,
                      const double* inv_ksq, double dx, double dy, double dvx, double dvy,
                      int physics_mode, double* Ex, double* Ey, double* delta_rho,
                      double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
+
+/* nsteps calls of vlasov_step_impl (the loop of Vlasov.run, P2:1392-1435)
+ * in place on f, with the X(dt/2) that closes one step and the X(dt/2) that
+ * opens the next fused into one pass; f after the call equals nsteps
+ * separate steps.  energy (optional, [nsteps]) receives the field energy
+ * 1/2 sum(Ex^2 + Ey^2) dx dy of every step's mid-step field; Ex, Ey,
+ * delta_rho, phi hold the last step's field.  Fused shapes only
+ * (vp_run_2d2v_workspace returns 0 otherwise). */
+size_t vp_run_2d2v_workspace(const int64_t shape[4]);
+int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, double dt, const double* vx,
+                    const double* vy, const double* kx, const double* ky, const double* inv_ksq,
+                    double dx, double dy, double dvx, double dvy, int physics_mode, double* Ex,
+                    double* Ey, double* delta_rho, double* phi, double* energy, void* ws,
+                    size_t ws_bytes, vp_stream_t stream);
+
+/* Kernel-level pieces of vp_run_2d2v_f64 / the fused vp_step_2d2v_f64, in
+ * the order they run (ws laid out as vp_run_2d2v_workspace); callers that
+ * time or interleave the kernels issue them one by one:
+ *   tables(dt/2)                                      once per (grid, dt)
+ *   xdrift(passes=1)                                  X(dt/2)
+ *   per step: ydrift(with_rho=1), poisson, kick(dt), ydrift(0),
+ *             xdrift(passes=2 between steps, 1 after the last)
+ * xdrift/ydrift/kick may run in place (f_in == f_out). */
+int vp_fused_tables_f64(const int64_t shape[4], const double* vx, const double* vy, double half_dt,
+                        double dx, double dy, void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_fused_xdrift_f64(const double* f_in, double* f_out, const int64_t shape[4], int passes,
+                        void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_fused_ydrift_f64(const double* f_in, double* f_out, const int64_t shape[4], int with_rho,
+                        void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_fused_poisson_f64(const int64_t shape[4], double dvx, double dvy, const double* kx,
+                         const double* ky, const double* inv_ksq, double* Ex, double* Ey,
+                         double* delta_rho, double* phi, void* ws, size_t ws_bytes,
+                         vp_stream_t stream);
+int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_t shape[4], const double* Ex,
+                      const double* Ey, double dt, double dvx, double dvy, vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Diagnostics (no reference counterpart; the reference only writes
+ * snapshots).
+ * --------------------------------------------------------------------- */
+
+/* out[0] = ((1/2 * sum_i (Ex[i]^2 + Ey[i]^2)) * dx) * dy, compensated;
+ * Ey may be NULL (1D: dy = 1). */
+int vp_field_energy_f64(const double* Ex, const double* Ey, int64_t n, double dx, double dy,
+                        double* out, vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Launch-shape knobs of the fused kernels (no reference counterpart).
+ * 0 = automatic.  Process-wide; used by the autotuner and the benchmarks.
+ * --------------------------------------------------------------------- */
+enum {
+    VP_KNOB_X_SPLIT = 0,       /* y-range splits of the X drift grid            */
+    VP_KNOB_Y_SPLIT = 1,       /* vx-range splits of the Y drift grid           */
+    VP_KNOB_KICK_PREFETCH = 2, /* 1: prefetch the next (vx,vy) plane in the kick */
+    VP_KNOB_COUNT = 3
+};
+int vp_tune_set(int knob, int value);
+int vp_tune_get(int knob);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,442 @@
+// Fused, in-place 2D2V sweeps for f[nx][ny][nvx][nvy] (float64).
+//
+// The generic sweeps (advect.hip) evaluate the departure point and the six
+// Lagrange weights per element.  Here the weights are hoisted:
+//
+//  * drift_table_kernel: the spatial drifts have time-constant velocities
+//    (vx for X, vy for Y), so the weights of every (row, velocity) pair are
+//    computed once per call with the exact reference arithmetic
+//    (vlasov2D2V.py:330-353) into a small table.
+//  * drift_kernel: a persistent workgroup owns one column chunk (32 vy) of
+//    a family of lines, keeps the weights of its (row, column) pairs in
+//    registers, and streams tile after tile (rows x 32 columns) through LDS
+//    with a one-tile register prefetch.  X drift (rows = x): the workgroup
+//    fixes (vx, vy chunk) and walks y, and can apply the drift twice in LDS
+//    (the X(dt/2) that ends one step and the X(dt/2) that starts the next).
+//    Y drift (rows = y): the workgroup fixes (x, vy chunk), walks vx and
+//    accumulates the charge density of its rows with Neumaier compensation.
+//  * kick_kernel: for one (x, y) the whole (vx, vy) plane is a closed unit
+//    for both kicks (vlasov2D2V.py:976-986), so one workgroup loads the plane
+//    into LDS once, applies the Vx kick then the Vy kick, and writes it back.
+//
+// Tiles and planes are disjoint between workgroups and each is read whole
+// before it is written, so all three kernels may run in place.
+#include "fast2d.hpp"
+
+namespace vp {
+
+constexpr int kCol = 32;       // columns (contiguous vy) per drift tile
+constexpr int kDriftThreads = 1024;
+constexpr int kGroups = kDriftThreads / kCol;  // row groups per drift tile
+
+// Weights and wrapped left index for every (row r, velocity index j):
+// W[(r*6 + k)*nvel + j], IL[r*nvel + j] = floor(r - (vel[j]*dt)/dx) mod n.
+__global__ void drift_table_kernel(const double* __restrict__ vel, int nvel, int n, double dt, double dx,
+                                   double* __restrict__ W, int* __restrict__ IL) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nvel * n) return;
+    const int r = idx / nvel;
+    const int j = idx - r * nvel;
+    const Departure d = departure(r, vel[j], dt, dx);
+    double w[6];
+    lagrange5_weights<double>(d.alpha, w);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) W[(static_cast<int64_t>(r) * 6 + k) * nvel + j] = w[k];
+    IL[static_cast<int64_t>(r) * nvel + j] = wrap_index(d.il, n);
+}
+
+template <int R>
+__device__ __forceinline__ double periodic_tap_sum(const double* tile, int il, const double (&w)[6], int n, int c) {
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int j = il + k - 2;
+        j += (j < 0) ? n : 0;
+        j -= (j >= n) ? n : 0;
+        v[k] = tile[j * kCol + c];
+    }
+    return tap_sum<double>(w, v);
+}
+
+// Persistent line sweep; see the file comment.  Layout of one workgroup:
+//   a  = fixed outer index (X: vx, Y: x),  q = vy chunk,  s = split of the loop
+//   element (row r, column c, loop l) = fin[a*a_stride + q*32 + c + r*row_stride + l*loop_stride]
+// LANE_W: the weight column is the vy index (Y drift) instead of a (X drift).
+template <int R, int PASSES, bool RHO, bool LANE_W>
+__global__ __launch_bounds__(kDriftThreads) void drift_kernel(
+    const double* fin, double* fout, int nq, int S, int64_t a_stride, int64_t row_stride,
+    int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
+    double2* __restrict__ partial) {
+    constexpr int n = R * kGroups;
+    __shared__ double tile[n * kCol];
+    const int t = threadIdx.x;
+    const int c = t & (kCol - 1);
+    const int g = t / kCol;
+    int b = blockIdx.x;
+    const int s = b % S;
+    b /= S;
+    const int q = b % nq;
+    const int a = b / nq;
+    const int64_t base = static_cast<int64_t>(a) * a_stride + q * kCol + c;
+    const int j = LANE_W ? q * kCol + c : a;
+
+    double w[R][6];
+    int il[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = g + kGroups * i;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[i][k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
+        il[i] = IL[static_cast<int64_t>(r) * nvel + j];
+    }
+    const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / S);
+    const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / S);
+    double cur[R];
+    KSum acc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        acc[i] = KSum{0.0, 0.0};
+        cur[i] = (l0 < l1) ? fin[base + (g + kGroups * i) * row_stride + l0 * loop_stride] : 0.0;
+    }
+    for (int l = l0; l < l1; ++l) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < R; ++i) tile[(g + kGroups * i) * kCol + c] = cur[i];
+        __syncthreads();
+        if (l + 1 < l1) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) cur[i] = fin[base + (g + kGroups * i) * row_stride + (l + 1) * loop_stride];
+        }
+        if (PASSES == 2) {
+            double out[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) out[i] = periodic_tap_sum<R>(tile, il[i], w[i], n, c);
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < R; ++i) tile[(g + kGroups * i) * kCol + c] = out[i];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const double o = periodic_tap_sum<R>(tile, il[i], w[i], n, c);
+            fout[base + (g + kGroups * i) * row_stride + l * loop_stride] = o;
+            if (RHO) ksum_add(acc[i], o);
+        }
+    }
+    if (RHO) {
+        const int nparts = nq * S;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            KSum k = acc[i];
+            for (int off = kCol / 2; off > 0; off >>= 1) {
+                KSum o;
+                o.s = __shfl_xor(k.s, off, kCol);
+                o.c = __shfl_xor(k.c, off, kCol);
+                ksum_merge(k, o);
+            }
+            if (c == 0) {
+                const int r = g + kGroups * i;
+                partial[(static_cast<int64_t>(a) * n + r) * nparts + q * S + s] = make_double2(k.s, k.c);
+            }
+        }
+    }
+}
+
+// Dirichlet tap (clamp + zero mask, vlasov2D2V.py:340-345).
+__device__ __forceinline__ double dirichlet_tap(const double* p, int j, int n, int stride) {
+    const int jc = j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
+    const double valid = (j >= 0 && j < n) ? 1.0 : 0.0;
+    return p[jc * stride] * valid;
+}
+
+// Vx kick then Vy kick on whole (vx, vy) planes; plane p = x*ny + y uses
+// Ex[p], Ey[p].  Thread t owns column c = t % NVY and the RR consecutive rows
+// starting at (t / NVY) * RR, so for NVY >= 64 one wave works on a single
+// row (the row's tap indices are wave-uniform) and row offsets fit immediates.
+// PREFETCH: load the next plane into registers while the Vy kick runs.
+template <int NVX, int NVY, bool PREFETCH>
+__global__ __launch_bounds__(1024) void kick_kernel(const double* fin, double* fout, int nplanes,
+                                                    const double* __restrict__ Ex, const double* __restrict__ Ey,
+                                                    double dt, double dvx, double dvy) {
+    constexpr int T = 1024;
+    constexpr int G = T / NVY;
+    constexpr int RR = NVX / G;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* P = reinterpret_cast<double*>(smem_raw);        // [NVX][NVY]
+    double* wx = P + NVX * NVY;                              // [NVX][6]
+    double* wy = wx + NVX * 6;                               // [NVY][6]
+    int* ilx = reinterpret_cast<int*>(wy + NVY * 6);         // [NVX]
+    int* ily = ilx + NVX;                                    // [NVY]
+    const int t = threadIdx.x;
+    const int c = t % NVY;
+    const int g = t / NVY;
+    constexpr int64_t plane = static_cast<int64_t>(NVX) * NVY;
+    double* Pt = P + g * RR * NVY + c;  // this thread's column slice of the plane
+
+    double cur[RR];
+    int p = blockIdx.x;
+    if (PREFETCH && p < nplanes) {
+#pragma unroll
+        for (int i = 0; i < RR; ++i) cur[i] = (fin + p * plane + g * RR * NVY + c)[i * NVY];
+    }
+    for (; p < nplanes; p += gridDim.x) {
+        __syncthreads();  // previous plane fully consumed
+        if (PREFETCH) {
+#pragma unroll
+            for (int i = 0; i < RR; ++i) Pt[i * NVY] = cur[i];
+        } else {
+            const double* src = fin + p * plane + g * RR * NVY + c;
+#pragma unroll
+            for (int i = 0; i < RR; ++i) Pt[i * NVY] = src[i * NVY];
+        }
+        if (t < NVX) {
+            const Departure d = departure(t, Ex[p], dt, dvx);
+            double w[6];
+            lagrange5_weights<double>(d.alpha, w);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wx[t * 6 + k] = w[k];
+            ilx[t] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
+        } else if (t < NVX + NVY) {
+            const int cc = t - NVX;
+            const Departure d = departure(cc, Ey[p], dt, dvy);
+            double w[6];
+            lagrange5_weights<double>(d.alpha, w);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wy[cc * 6 + k] = w[k];
+            ily[cc] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVY + 8 ? NVY + 8 : d.il));
+        }
+        __syncthreads();
+        double out[RR];
+        // Vx kick: line along vx (stride NVY); row r is uniform over the wave.
+#pragma unroll
+        for (int i = 0; i < RR; ++i) {
+            const int r = g * RR + i;
+            // a wave covers one row when NVY >= 64: the index is then scalar
+            const int l = NVY >= kWave ? __builtin_amdgcn_readfirstlane(ilx[r]) : ilx[r];
+            double w[6], v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                w[k] = wx[r * 6 + k];
+                const int j = l + k - 2;  // wave-uniform: scalar clamp and mask
+                const int jc = j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j);
+                const double m = (j >= 0 && j < NVX) ? 1.0 : 0.0;
+                v[k] = P[jc * NVY + c] * m;
+            }
+            out[i] = tap_sum<double>(w, v);
+            __builtin_amdgcn_sched_barrier(0);  // keep rows apart: bounds the live registers
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < RR; ++i) Pt[i * NVY] = out[i];
+        __syncthreads();
+        const int pn = p + gridDim.x;
+        if (PREFETCH && pn < nplanes) {
+#pragma unroll
+            for (int i = 0; i < RR; ++i) cur[i] = (fin + pn * plane + g * RR * NVY + c)[i * NVY];
+        }
+        // Vy kick: contiguous line, weights and clamped taps of column c.
+        double w[6], m[6];
+        int jy[6];
+        const int l = ily[c];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            w[k] = wy[c * 6 + k];
+            const int j = l + k - 2;
+            jy[k] = j < 0 ? 0 : (j > NVY - 1 ? NVY - 1 : j);
+            m[k] = (j >= 0 && j < NVY) ? 1.0 : 0.0;
+        }
+        double* dst = fout + p * plane + g * RR * NVY + c;
+        const double* Prow = P + g * RR * NVY;
+#pragma unroll
+        for (int i = 0; i < RR; ++i) {
+            double v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = Prow[i * NVY + jy[k]] * m[k];
+            dst[i * NVY] = tap_sum<double>(w, v);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+// Launch-shape knobs (vp_tune_set); 0 = automatic choice.
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 1};
+
+int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
+
+static int cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cached[dev] = v;
+    }
+    return cached[dev];
+}
+
+bool fast2d_supported(const int64_t shape[4]) {
+    const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
+    auto ok = [](int64_t n) { return n == 32 || n == 64 || n == 128; };
+    return ok(nx) && ok(ny) && ok(nvx) && ok(nvy);
+}
+
+static inline size_t a256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+static int y_split(int64_t nx, int64_t nvx, int64_t nvy) {
+    int S = knob(VP_KNOB_Y_SPLIT);
+    if (S <= 0) {
+        // enough workgroups to cover the chip at one 1024-thread workgroup per CU
+        const int64_t base = nx * (nvy / kCol);
+        S = 1;
+        while (base * S < 2 * 256 && S < 8) S *= 2;
+    }
+    if (S > nvx) S = static_cast<int>(nvx);
+    return S;
+}
+
+Fast2DWs fast2d_layout(const int64_t shape[4], void* ws) {
+    const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
+    Fast2DWs L{};
+    unsigned char* p = static_cast<unsigned char*>(ws);
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        unsigned char* q = p ? p + off : nullptr;
+        off += a256(b);
+        return q;
+    };
+    L.Wx = reinterpret_cast<double*>(take(static_cast<size_t>(nx) * 6 * nvx * sizeof(double)));
+    L.ILx = reinterpret_cast<int*>(take(static_cast<size_t>(nx) * nvx * sizeof(int)));
+    L.Wy = reinterpret_cast<double*>(take(static_cast<size_t>(ny) * 6 * nvy * sizeof(double)));
+    L.ILy = reinterpret_cast<int*>(take(static_cast<size_t>(ny) * nvy * sizeof(int)));
+    L.nparts = static_cast<int>((nvy / kCol) * y_split(nx, nvx, nvy));
+    L.partial = reinterpret_cast<double2*>(take(static_cast<size_t>(nx) * ny * L.nparts * sizeof(double2)));
+    L.bytes = off;
+    return L;
+}
+
+int fast2d_tables(const int64_t shape[4], const Fast2DWs& L, const double* vx, const double* vy, double half_dt,
+                  double dx, double dy, hipStream_t s) {
+    const int nx = static_cast<int>(shape[0]), ny = static_cast<int>(shape[1]);
+    const int nvx = static_cast<int>(shape[2]), nvy = static_cast<int>(shape[3]);
+    hipLaunchKernelGGL(drift_table_kernel, dim3((nx * nvx + 255) / 256), dim3(256), 0, s, vx, nvx, nx, half_dt, dx,
+                       L.Wx, L.ILx);
+    VP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(drift_table_kernel, dim3((ny * nvy + 255) / 256), dim3(256), 0, s, vy, nvy, ny, half_dt, dy,
+                       L.Wy, L.ILy);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+// X drift (passes = 1 or 2) from fin to fout (may alias).
+int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, int passes,
+                  hipStream_t s) {
+    const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
+    const int nq = static_cast<int>(nvy / kCol);
+    int S = knob(VP_KNOB_X_SPLIT);
+    if (S <= 0) {
+        S = 1;
+        while (nvx * nq * S < 4 * 256 && S < static_cast<int>(ny)) S *= 2;
+    }
+    if (S > static_cast<int>(ny)) S = static_cast<int>(ny);
+    const int64_t blocks = nvx * nq * S;
+    const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
+    const int nvel = static_cast<int>(nvx), lt = static_cast<int>(ny);
+#define VP_X(RV, PV)                                                                                             \
+    hipLaunchKernelGGL((drift_kernel<RV, PV, false, false>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, \
+                       nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel, nullptr)
+    if (nx == 128) {
+        if (passes == 2) VP_X(4, 2); else VP_X(4, 1);
+    } else if (nx == 64) {
+        if (passes == 2) VP_X(2, 2); else VP_X(2, 1);
+    } else if (nx == 32) {
+        if (passes == 2) VP_X(1, 2); else VP_X(1, 1);
+    } else {
+        return VP_ERR_UNSUPPORTED;
+    }
+#undef VP_X
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+// Y drift (+ rho partials when rho != 0) from fin to fout (may alias).
+int fast2d_ydrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, bool rho,
+                  hipStream_t s) {
+    const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
+    const int nq = static_cast<int>(nvy / kCol);
+    const int S = y_split(nx, nvx, nvy);
+    const int64_t blocks = nx * nq * S;
+    const int64_t a_stride = ny * nvx * nvy, row_stride = nvx * nvy, loop_stride = nvy;
+    const int nvel = static_cast<int>(nvy), lt = static_cast<int>(nvx);
+#define VP_Y(RV, RH)                                                                                              \
+    hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, nq, S, \
+                       a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel, L.partial)
+    if (ny == 128) {
+        if (rho) VP_Y(4, true); else VP_Y(4, false);
+    } else if (ny == 64) {
+        if (rho) VP_Y(2, true); else VP_Y(2, false);
+    } else if (ny == 32) {
+        if (rho) VP_Y(1, true); else VP_Y(1, false);
+    } else {
+        return VP_ERR_UNSUPPORTED;
+    }
+#undef VP_Y
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+template <int NVX, int NVY, bool PF>
+static int launch_kick(const double* fin, double* fout, int nplanes, const double* Ex, const double* Ey, double dt,
+                       double dvx, double dvy, hipStream_t s) {
+    const size_t lds = static_cast<size_t>(NVX) * NVY * sizeof(double) + (NVX + NVY) * 6 * sizeof(double) +
+                       (NVX + NVY) * sizeof(int);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&kick_kernel<NVX, NVY, PF>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return static_cast<int>(e);
+        attr_set = true;
+    }
+    const int per_cu = lds > 81920 ? 1 : (lds > 54000 ? 2 : (lds > 40000 ? 3 : 4));
+    int grid = cu_count() * per_cu;
+    if (grid > nplanes) grid = nplanes;
+    hipLaunchKernelGGL((kick_kernel<NVX, NVY, PF>), dim3(grid), dim3(1024), lds, s, fin, fout, nplanes, Ex, Ey, dt, dvx,
+                       dvy);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const double* Ex, const double* Ey,
+                double dt, double dvx, double dvy, hipStream_t s) {
+    const int64_t nvx = shape[2], nvy = shape[3];
+    const int nplanes = static_cast<int>(shape[0] * shape[1]);
+    const bool pf = knob(VP_KNOB_KICK_PREFETCH) != 0;
+#define VP_K(A, B)                                                                                  \
+    if (nvx == A && nvy == B)                                                                       \
+        return pf ? launch_kick<A, B, true>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s)            \
+                  : launch_kick<A, B, false>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+    VP_K(128, 128)
+    VP_K(128, 64)
+    VP_K(128, 32)
+    VP_K(64, 128)
+    VP_K(64, 64)
+    VP_K(64, 32)
+    VP_K(32, 128)
+    VP_K(32, 64)
+    VP_K(32, 32)
+#undef VP_K
+    return VP_ERR_UNSUPPORTED;
+}
+
+}  // namespace vp
+
+extern "C" int vp_tune_set(int knob, int value) {
+    if (knob < 0 || knob >= VP_KNOB_COUNT) return VP_ERR_ARG;
+    vp::g_knobs[knob] = value;
+    return VP_OK;
+}
+
+extern "C" int vp_tune_get(int knob) { return vp::knob(knob); }

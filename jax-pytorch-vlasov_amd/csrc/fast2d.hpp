@@ -1,0 +1,31 @@
+// Host interface of the fused in-place 2D2V sweeps (fast2d.hip).
+#pragma once
+
+#include "vp_common.hpp"
+
+namespace vp {
+
+// Workspace of the fused path: X/Y weight tables and the rho partial sums.
+struct Fast2DWs {
+    double* Wx;
+    int* ILx;
+    double* Wy;
+    int* ILy;
+    double2* partial;
+    int nparts;
+    size_t bytes;
+};
+
+bool fast2d_supported(const int64_t shape[4]);
+// Carves the workspace (ws == nullptr: only computes .bytes).
+Fast2DWs fast2d_layout(const int64_t shape[4], void* ws);
+int fast2d_tables(const int64_t shape[4], const Fast2DWs& L, const double* vx, const double* vy, double half_dt,
+                  double dx, double dy, hipStream_t s);
+int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, int passes,
+                  hipStream_t s);
+int fast2d_ydrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, bool rho,
+                  hipStream_t s);
+int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const double* Ex, const double* Ey,
+                double dt, double dvx, double dvy, hipStream_t s);
+
+}  // namespace vp

@@ -84,7 +84,10 @@ __global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restric
 // 2D: compute_poisson_real (vlasov2D2V.py:488-517).
 // ws: R[nx][nky], T[3][nx][nky]  (double2), nky = ny/2 + 1
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ rho, int nx, int ny,
+// rho comes either from `rho` or, when `partial` is given, from nparts
+// compensated partial sums per point (merged in order) times dvx, then dvy.
+__global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ rho, const double2* __restrict__ partial,
+                                                    int nparts, double dvx, double dvy, int nx, int ny,
                                                     double* __restrict__ delta_rho, double2* __restrict__ R) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double2* tw = reinterpret_cast<double2*>(smem_raw);
@@ -92,7 +95,16 @@ __global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ r
     const int x = blockIdx.x;
     fill_twiddles(tw, ny);
     for (int y = threadIdx.x; y < ny; y += blockDim.x) {
-        const double d = rho[static_cast<int64_t>(x) * ny + y] - 1.0;
+        double rv;
+        if (partial) {
+            const double2* pp = partial + (static_cast<int64_t>(x) * ny + y) * nparts;
+            KSum k{0.0, 0.0};
+            for (int i = 0; i < nparts; ++i) ksum_merge(k, KSum{pp[i].x, pp[i].y});
+            rv = ((k.s + k.c) * dvx) * dvy;
+        } else {
+            rv = rho[static_cast<int64_t>(x) * ny + y];
+        }
+        const double d = rv - 1.0;
         row[y] = d;
         if (delta_rho) delta_rho[static_cast<int64_t>(x) * ny + y] = d;
     }
@@ -224,24 +236,57 @@ size_t poisson2d_ws(int64_t nx, int64_t ny) {
     return static_cast<size_t>(4) * nx * (ny / 2 + 1) * sizeof(double2);
 }
 
-int poisson2d(const double* rho, int64_t nx, int64_t ny, const double* kx, const double* ky,
-              const double* inv_ksq, double* Ex, double* Ey, double* delta_rho, double* phi, void* ws,
-              size_t ws_bytes, hipStream_t s) {
-    VP_RETURN_IF(!rho || !kx || !ky || !inv_ksq || !ws, VP_ERR_ARG);
+int poisson2d_from(const double* rho, const double2* partial, int nparts, double dvx, double dvy, int64_t nx,
+                   int64_t ny, const double* kx, const double* ky, const double* inv_ksq, double* Ex, double* Ey,
+                   double* delta_rho, double* phi, void* ws, size_t ws_bytes, hipStream_t s) {
+    VP_RETURN_IF((!rho && !partial) || !kx || !ky || !inv_ksq || !ws, VP_ERR_ARG);
     VP_RETURN_IF(nx < 1 || ny < 1 || nx > 512 || ny > 512, VP_ERR_SHAPE);
     VP_RETURN_IF(ws_bytes < poisson2d_ws(nx, ny), VP_ERR_WORKSPACE);
     const int nxi = static_cast<int>(nx), nyi = static_cast<int>(ny);
     const int nky = nyi / 2 + 1;
     double2* R = static_cast<double2*>(ws);
     double2* T = R + static_cast<int64_t>(nx) * nky;
-    hipLaunchKernelGGL(p2d_rows_fwd, dim3(nxi), dim3(256), ny * (sizeof(double2) + sizeof(double)), s, rho,
-                       nxi, nyi, delta_rho, R);
+    hipLaunchKernelGGL(p2d_rows_fwd, dim3(nxi), dim3(256), ny * (sizeof(double2) + sizeof(double)), s, rho, partial,
+                       nparts, dvx, dvy, nxi, nyi, delta_rho, R);
     VP_CHECK_LAUNCH();
     hipLaunchKernelGGL(p2d_cols, dim3(nky), dim3(256), 5 * nx * sizeof(double2), s, R, nxi, nyi, kx, ky,
                        inv_ksq, T);
     VP_CHECK_LAUNCH();
     hipLaunchKernelGGL(p2d_rows_inv, dim3(3 * nxi), dim3(256), (ny + nky) * sizeof(double2), s, T, nxi, nyi,
                        phi, Ex, Ey);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+int poisson2d(const double* rho, int64_t nx, int64_t ny, const double* kx, const double* ky,
+              const double* inv_ksq, double* Ex, double* Ey, double* delta_rho, double* phi, void* ws,
+              size_t ws_bytes, hipStream_t s) {
+    VP_RETURN_IF(!rho, VP_ERR_ARG);
+    return poisson2d_from(rho, nullptr, 0, 1.0, 1.0, nx, ny, kx, ky, inv_ksq, Ex, Ey, delta_rho, phi, ws, ws_bytes,
+                          s);
+}
+
+// out[0] = ((0.5 * sum(a^2 + b^2)) * s1) * s2  (b may be NULL), compensated.
+__global__ __launch_bounds__(1024) void energy_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                      int64_t n, double s1, double s2, double* __restrict__ out) {
+    __shared__ double scratch[2 * 1024 / kWave];
+    KSum k{0.0, 0.0};
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double av = a[i];
+        double e = av * av;
+        if (b) {
+            const double bv = b[i];
+            e = e + bv * bv;
+        }
+        ksum_add(k, e);
+    }
+    const double total = block_ksum(k, scratch);
+    if (threadIdx.x == 0) out[0] = ((0.5 * total) * s1) * s2;
+}
+
+int field_energy(const double* a, const double* b, int64_t n, double s1, double s2, double* out, hipStream_t s) {
+    VP_RETURN_IF(!a || !out || n < 1, VP_ERR_ARG);
+    hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(1024), 0, s, a, b, n, s1, s2, out);
     VP_CHECK_LAUNCH();
     return VP_OK;
 }

@@ -1,6 +1,6 @@
 // Full Strang steps composed from the sweep and field kernels, plus the
 // small C-ABI utilities (error strings, version).
-#include "vp_common.hpp"
+#include "fast2d.hpp"
 
 namespace vp {
 int fill(double* p, int64_t n, double v, hipStream_t s);
@@ -14,6 +14,11 @@ size_t poisson2d_ws(int64_t nx, int64_t ny);
 int poisson2d(const double* rho, int64_t nx, int64_t ny, const double* kx, const double* ky,
               const double* inv_ksq, double* Ex, double* Ey, double* delta_rho, double* phi, void* ws,
               size_t ws_bytes, hipStream_t s);
+
+int poisson2d_from(const double* rho, const double2* partial, int nparts, double dvx, double dvy, int64_t nx,
+                   int64_t ny, const double* kx, const double* ky, const double* inv_ksq, double* Ex, double* Ey,
+                   double* delta_rho, double* phi, void* ws, size_t ws_bytes, hipStream_t s);
+int field_energy(const double* a, const double* b, int64_t n, double s1, double s2, double* out, hipStream_t s);
 
 static inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
@@ -82,13 +87,149 @@ extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, i
 
 // ---------------------------------------------------------------------------
 // 2D2V: vlasov_step_impl (vlasov2D2V.py:943-998)
-// ws: [N^4] f scratch | [nx*ny] rho | poisson ws
+// generic ws: [N^4] f scratch | [nx*ny] rho | poisson ws
+// fused ws:   weight tables | rho partials | poisson ws   (sweeps run in place)
 // ---------------------------------------------------------------------------
-extern "C" size_t vp_step_2d2v_workspace(const int64_t shape[4]) {
+static size_t generic_2d_ws(const int64_t shape[4]) {
     const size_t cells = static_cast<size_t>(shape[0]) * shape[1] * shape[2] * shape[3];
     return vp::align256(cells * sizeof(double)) +
            vp::align256(static_cast<size_t>(shape[0]) * shape[1] * sizeof(double)) +
            vp::poisson2d_ws(shape[0], shape[1]);
+}
+
+static size_t fused_2d_ws(const int64_t shape[4]) {
+    return vp::fast2d_layout(shape, nullptr).bytes + vp::poisson2d_ws(shape[0], shape[1]);
+}
+
+extern "C" size_t vp_step_2d2v_workspace(const int64_t shape[4]) {
+    if (!shape) return 0;
+    return vp::fast2d_supported(shape) ? fused_2d_ws(shape) : generic_2d_ws(shape);
+}
+
+extern "C" size_t vp_run_2d2v_workspace(const int64_t shape[4]) {
+    if (!shape || !vp::fast2d_supported(shape)) return 0;
+    return fused_2d_ws(shape);
+}
+
+extern "C" int vp_field_energy_f64(const double* Ex, const double* Ey, int64_t n, double dx, double dy,
+                                   double* out, vp_stream_t stream) {
+    return vp::field_energy(Ex, Ey, n, dx, dy, out, static_cast<hipStream_t>(stream));
+}
+
+// Fused step: X (f_in -> f_out), then Y+rho, Poisson, kick, Y, X in place on f_out.
+static int fused_step_2d(const double* f_in, double* f_out, const int64_t shape[4], double dt, const double* vx,
+                         const double* vy, const double* kx, const double* ky, const double* inv_ksq, double dx,
+                         double dy, double dvx, double dvy, int physics_mode, double* Ex, double* Ey,
+                         double* delta_rho, double* phi, void* ws, hipStream_t s) {
+    const int64_t nx = shape[0], ny = shape[1];
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws);
+    void* pws = static_cast<unsigned char*>(ws) + L.bytes;
+    const double half = dt / 2.0;
+    VP_TRY(vp::fast2d_tables(shape, L, vx, vy, half, dx, dy, s));
+    VP_TRY(vp::fast2d_xdrift(f_in, f_out, shape, L, 1, s));
+    VP_TRY(vp::fast2d_ydrift(f_out, f_out, shape, L, physics_mode != 0, s));
+    if (physics_mode) {
+        VP_TRY(vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, nx, ny, kx, ky, inv_ksq, Ex, Ey,
+                                  delta_rho, phi, pws, vp::poisson2d_ws(nx, ny), s));
+    } else {
+        VP_TRY(vp::fill(Ex, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(Ey, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(delta_rho, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(phi, nx * ny, 0.0, s));
+    }
+    VP_TRY(vp::fast2d_kick(f_out, f_out, shape, Ex, Ey, dt, dvx, dvy, s));
+    VP_TRY(vp::fast2d_ydrift(f_out, f_out, shape, L, false, s));
+    VP_TRY(vp::fast2d_xdrift(f_out, f_out, shape, L, 1, s));
+    return VP_OK;
+}
+
+// Kernel-level pieces of the fused step, in the order vp_run_2d2v_f64 issues
+// them; ws is laid out as for vp_run_2d2v_workspace.
+#define VP_FUSED_GUARD(ws, ws_bytes)                                   \
+    VP_RETURN_IF(!shape || !ws, VP_ERR_ARG);                           \
+    VP_RETURN_IF(!vp::fast2d_supported(shape), VP_ERR_UNSUPPORTED);    \
+    VP_RETURN_IF(ws_bytes < fused_2d_ws(shape), VP_ERR_WORKSPACE)
+
+extern "C" int vp_fused_tables_f64(const int64_t shape[4], const double* vx, const double* vy, double half_dt,
+                                   double dx, double dy, void* ws, size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD(ws, ws_bytes);
+    VP_RETURN_IF(!vx || !vy, VP_ERR_ARG);
+    return vp::fast2d_tables(shape, vp::fast2d_layout(shape, ws), vx, vy, half_dt, dx, dy,
+                             static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_xdrift_f64(const double* f_in, double* f_out, const int64_t shape[4], int passes, void* ws,
+                                   size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD(ws, ws_bytes);
+    VP_RETURN_IF(!f_in || !f_out || (passes != 1 && passes != 2), VP_ERR_ARG);
+    return vp::fast2d_xdrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws), passes,
+                             static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_ydrift_f64(const double* f_in, double* f_out, const int64_t shape[4], int with_rho,
+                                   void* ws, size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD(ws, ws_bytes);
+    VP_RETURN_IF(!f_in || !f_out, VP_ERR_ARG);
+    return vp::fast2d_ydrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws), with_rho != 0,
+                             static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_poisson_f64(const int64_t shape[4], double dvx, double dvy, const double* kx,
+                                    const double* ky, const double* inv_ksq, double* Ex, double* Ey,
+                                    double* delta_rho, double* phi, void* ws, size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD(ws, ws_bytes);
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws);
+    void* pws = static_cast<unsigned char*>(ws) + L.bytes;
+    return vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, shape[0], shape[1], kx, ky, inv_ksq, Ex, Ey,
+                              delta_rho, phi, pws, vp::poisson2d_ws(shape[0], shape[1]),
+                              static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_t shape[4], const double* Ex,
+                                 const double* Ey, double dt, double dvx, double dvy, vp_stream_t stream) {
+    VP_RETURN_IF(!shape || !f_in || !f_out || !Ex || !Ey, VP_ERR_ARG);
+    VP_RETURN_IF(!vp::fast2d_supported(shape), VP_ERR_UNSUPPORTED);
+    return vp::fast2d_kick(f_in, f_out, shape, Ex, Ey, dt, dvx, dvy, static_cast<hipStream_t>(stream));
+}
+
+// nsteps Strang steps in place on f.  Between two steps the closing X(dt/2)
+// and the opening X(dt/2) run in one pass (applied twice in LDS): the
+// result is identical to nsteps separate steps.
+extern "C" int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, double dt, const double* vx,
+                               const double* vy, const double* kx, const double* ky, const double* inv_ksq,
+                               double dx, double dy, double dvx, double dvy, int physics_mode, double* Ex,
+                               double* Ey, double* delta_rho, double* phi, double* energy, void* ws,
+                               size_t ws_bytes, vp_stream_t stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VP_RETURN_IF(!f || !shape || !vx || !vy || !Ex || !Ey || !delta_rho || !phi || !ws, VP_ERR_ARG);
+    VP_RETURN_IF(physics_mode && (!kx || !ky || !inv_ksq), VP_ERR_ARG);
+    VP_RETURN_IF(nsteps < 0, VP_ERR_ARG);
+    VP_RETURN_IF(!vp::fast2d_supported(shape), VP_ERR_UNSUPPORTED);
+    VP_RETURN_IF(ws_bytes < fused_2d_ws(shape), VP_ERR_WORKSPACE);
+    if (nsteps == 0) return VP_OK;
+    const int64_t nx = shape[0], ny = shape[1];
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws);
+    void* pws = static_cast<unsigned char*>(ws) + L.bytes;
+    const double half = dt / 2.0;
+    VP_TRY(vp::fast2d_tables(shape, L, vx, vy, half, dx, dy, s));
+    if (!physics_mode) {
+        VP_TRY(vp::fill(Ex, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(Ey, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(delta_rho, nx * ny, 0.0, s));
+        VP_TRY(vp::fill(phi, nx * ny, 0.0, s));
+    }
+    VP_TRY(vp::fast2d_xdrift(f, f, shape, L, 1, s));
+    for (int st = 0; st < nsteps; ++st) {
+        VP_TRY(vp::fast2d_ydrift(f, f, shape, L, physics_mode != 0, s));
+        if (physics_mode)
+            VP_TRY(vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, nx, ny, kx, ky, inv_ksq, Ex, Ey,
+                                      delta_rho, phi, pws, vp::poisson2d_ws(nx, ny), s));
+        if (energy) VP_TRY(vp::field_energy(Ex, Ey, nx * ny, dx, dy, energy + st, s));
+        VP_TRY(vp::fast2d_kick(f, f, shape, Ex, Ey, dt, dvx, dvy, s));
+        VP_TRY(vp::fast2d_ydrift(f, f, shape, L, false, s));
+        VP_TRY(vp::fast2d_xdrift(f, f, shape, L, st == nsteps - 1 ? 1 : 2, s));
+    }
+    return VP_OK;
 }
 
 extern "C" int vp_step_2d2v_f64(const double* f_in, double* f_out, const int64_t shape[4], double dt,
@@ -101,6 +242,9 @@ extern "C" int vp_step_2d2v_f64(const double* f_in, double* f_out, const int64_t
                  VP_ERR_ARG);
     VP_RETURN_IF(physics_mode && (!kx || !ky || !inv_ksq), VP_ERR_ARG);
     VP_RETURN_IF(ws_bytes < vp_step_2d2v_workspace(shape), VP_ERR_WORKSPACE);
+    if (vp::fast2d_supported(shape))
+        return fused_step_2d(f_in, f_out, shape, dt, vx, vy, kx, ky, inv_ksq, dx, dy, dvx, dvy, physics_mode, Ex, Ey,
+                             delta_rho, phi, ws, s);
     const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
     const size_t cells = static_cast<size_t>(nx) * ny * nvx * nvy;
     unsigned char* wb = static_cast<unsigned char*>(ws);

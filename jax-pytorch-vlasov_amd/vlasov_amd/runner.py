@@ -1,10 +1,15 @@
 """Allocation-free multi-step drivers.
 
-``Run2D`` / ``Run1D`` own two ping-pong copies of f, the field arrays and
-the workspace, and advance them in place step after step: no tensor is
-allocated inside ``step()``.  Each step issues the same kernels as the
-functional ``vlasov_step_impl`` / ``vlasov_step`` (bit-identical results),
-one C-ABI call per kernel so a caller can bracket any of them with events.
+``Run2D`` owns the state f, the field arrays and the workspace and advances
+them step after step without allocating.  Two kernel paths:
+
+* fused (nx, ny, nvx, nvy in {64, 128}): in-place kernels, one C-ABI call
+  per kernel (so a caller can bracket any of them with events), the closing
+  X(dt/2) of a step and the opening X(dt/2) of the next run as one pass;
+* generic (any shape): the six separate sweeps of the reference step,
+  ping-ponging between two copies of f.
+
+Both paths give the same f as repeated calls of ``vlasov_step_impl``.
 """
 from __future__ import annotations
 
@@ -13,25 +18,33 @@ import torch
 from . import _lib
 from ._lib import check, lib, stream_handle
 
-_KERNELS_2D = ("x_drift", "y_drift", "rho", "poisson", "vx_kick", "vy_kick", "y_drift2", "x_drift2")
+FUSED_KERNELS = ("x_drift", "y_drift_rho", "poisson", "kick", "y_drift", "xx_drift", "field_energy")
+GENERIC_KERNELS = ("x_drift", "y_drift", "rho", "poisson", "vx_kick", "vy_kick", "y_drift2", "x_drift2",
+                   "field_energy")
+
+
+def fused_supported(shape) -> bool:
+    return all(int(n) in (64, 128) for n in shape)
 
 
 class Run2D:
-    """In-place 2D2V Strang stepping on f[Nx, Ny, Nvx, Nvy] (float64)."""
+    """In-place 2D2V Strang stepping of f[Nx, Ny, Nvx, Nvy] (float64)."""
 
-    def __init__(self, grid, f: torch.Tensor, dt: float, physics_mode: bool = True):
+    def __init__(self, grid, f: torch.Tensor, dt: float, physics_mode: bool = True, fused: bool | None = None):
         self.g = grid
         self.dt = float(dt)
         self.physics_mode = bool(physics_mode)
         dev = f.device
         self.device = dev
-        self.shape = tuple(f.shape)
+        self.shape = tuple(int(n) for n in f.shape)
+        self.fused = fused_supported(self.shape) if fused is None else bool(fused)
+        if self.fused and not fused_supported(self.shape):
+            raise ValueError(f"the fused path needs every extent in {{64, 128}}, got {self.shape}")
         Nx, Ny = self.shape[0], self.shape[1]
-        self.f = [f.contiguous().clone(), torch.empty_like(f)]
-        self.cur = 0
-        self.fields = torch.zeros((4, Nx, Ny), dtype=torch.float64, device=dev)  # Ex, Ey, drho, phi
-        self.rho = torch.empty((Nx, Ny), dtype=torch.float64, device=dev)
-        self.pws = torch.empty(lib().vp_poisson_2d_workspace(Nx, Ny), dtype=torch.uint8, device=dev)
+        self.f = [f.contiguous().clone()]
+        if not self.fused:
+            self.f.append(torch.empty_like(self.f[0]))
+        self.fields = torch.zeros((4, Nx, Ny), dtype=torch.float64, device=dev)  # Ex, Ey, delta_rho, phi
         f64 = torch.float64
         self.vx = grid.vx.to(f64).contiguous()
         self.vy = grid.vy.to(f64).contiguous()
@@ -39,13 +52,20 @@ class Run2D:
         self.ky = grid.ky.to(f64).contiguous()
         self.iks = grid.inv_ksq.to(f64).contiguous()
         self.c_shape = _lib.i64x4(self.shape)
-        self.vs_x = _lib.i64x4((0, 0, 1, 0))
-        self.vs_y = _lib.i64x4((0, 0, 0, 1))
-        self.vs_e = _lib.i64x4((Ny, 1, 0, 0))
+        if self.fused:
+            self.ws = torch.empty(lib().vp_run_2d2v_workspace(self.c_shape), dtype=torch.uint8, device=dev)
+            self._tables_ready = False
+        else:
+            self.rho = torch.empty((Nx, Ny), dtype=f64, device=dev)
+            self.pws = torch.empty(lib().vp_poisson_2d_workspace(Nx, Ny), dtype=torch.uint8, device=dev)
+            self.vs_x = _lib.i64x4((0, 0, 1, 0))
+            self.vs_y = _lib.i64x4((0, 0, 0, 1))
+            self.vs_e = _lib.i64x4((Ny, 1, 0, 0))
+        self.steps_done = 0
 
     @property
     def state(self) -> torch.Tensor:
-        return self.f[self.cur]
+        return self.f[0]
 
     @property
     def Ex(self):
@@ -63,45 +83,95 @@ class Run2D:
     def phi(self):
         return self.fields[3]
 
+    @staticmethod
+    def _timed(events, name, fn):
+        lst = None if events is None else events.get(name)
+        if lst is None:
+            fn()
+            return
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        lst.append((e0, e1))
+
+    def advance(self, nsteps: int, events: dict | None = None, energy: torch.Tensor | None = None) -> None:
+        """Advance ``nsteps`` steps.  ``events`` maps kernel names (FUSED_KERNELS
+        or GENERIC_KERNELS) to lists receiving (start, end) event pairs;
+        ``energy`` (float64 [>= nsteps], device) receives the field energy of
+        each step."""
+        if nsteps <= 0:
+            return
+        if self.fused:
+            self._advance_fused(nsteps, events, energy)
+        else:
+            for st in range(nsteps):
+                self._step_generic(events, None if energy is None else energy[st:st + 1])
+        self.steps_done += nsteps
+
+    def _advance_fused(self, nsteps, events, energy):
+        g = self.g
+        s = stream_handle(self.device)
+        L = lib()
+        f = self.f[0]
+        fp = f.data_ptr()
+        ws, wsn = self.ws.data_ptr(), self.ws.numel()
+        Ex, Ey, drho, phi = (t.data_ptr() for t in self.fields)
+        sh = self.c_shape
+        if not self._tables_ready:
+            check(L.vp_fused_tables_f64(sh, self.vx.data_ptr(), self.vy.data_ptr(), self.dt / 2.0, g.dx, g.dy, ws,
+                                        wsn, s), "vp_fused_tables_f64")
+            self._tables_ready = True
+        if not self.physics_mode:
+            self.fields.zero_()
+        t = self._timed
+        t(events, "x_drift", lambda: check(L.vp_fused_xdrift_f64(fp, fp, sh, 1, ws, wsn, s), "xdrift"))
+        for st in range(nsteps):
+            t(events, "y_drift_rho", lambda: check(L.vp_fused_ydrift_f64(fp, fp, sh, int(self.physics_mode), ws,
+                                                                         wsn, s), "ydrift"))
+            if self.physics_mode:
+                t(events, "poisson", lambda: check(L.vp_fused_poisson_f64(
+                    sh, g.dvx, g.dvy, self.kx.data_ptr(), self.ky.data_ptr(), self.iks.data_ptr(), Ex, Ey, drho,
+                    phi, ws, wsn, s), "poisson"))
+            if energy is not None:
+                ep = energy.data_ptr() + 8 * st
+                t(events, "field_energy", lambda: check(L.vp_field_energy_f64(
+                    Ex, Ey, self.shape[0] * self.shape[1], g.dx, g.dy, ep, s), "field_energy"))
+            t(events, "kick", lambda: check(L.vp_fused_kick_f64(fp, fp, sh, Ex, Ey, self.dt, g.dvx, g.dvy, s),
+                                            "kick"))
+            t(events, "y_drift", lambda: check(L.vp_fused_ydrift_f64(fp, fp, sh, 0, ws, wsn, s), "ydrift"))
+            last = st == nsteps - 1
+            t(events, "x_drift" if last else "xx_drift",
+              lambda: check(L.vp_fused_xdrift_f64(fp, fp, sh, 1 if last else 2, ws, wsn, s), "xdrift"))
+
     def _adv(self, src, dst, axis, vel, vs, dt, dx, periodic, s):
         check(lib().vp_advect_f64(src.data_ptr(), dst.data_ptr(), self.c_shape, axis, vel.data_ptr(), vs,
                                   dt, dx, periodic, s), "vp_advect_f64")
 
-    def step(self, events: dict | None = None) -> None:
-        """Advance one step.  ``events`` maps a kernel name in
-        ("x_drift", "y_drift", "rho", "poisson", "vx_kick", "vy_kick", "y_drift2", "x_drift2")
-        to a list that receives (start, end) torch.cuda.Event pairs."""
+    def _step_generic(self, events, energy):
         g = self.g
         s = stream_handle(self.device)
-        a, b = self.f[self.cur], self.f[1 - self.cur]
+        a, b = self.f[0], self.f[1]
         half = self.dt / 2.0
         Nx, Ny, Nvx, Nvy = self.shape
-        ev = events or {}
-
-        def run(name, fn):
-            lst = ev.get(name)
-            if lst is None:
-                fn()
-                return
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            fn()
-            e1.record()
-            lst.append((e0, e1))
-
-        run("x_drift", lambda: self._adv(a, b, 0, self.vx, self.vs_x, half, g.dx, 1, s))
-        run("y_drift", lambda: self._adv(b, a, 1, self.vy, self.vs_y, half, g.dy, 1, s))
+        t = self._timed
         Ex, Ey, drho, phi = self.fields[0], self.fields[1], self.fields[2], self.fields[3]
+        t(events, "x_drift", lambda: self._adv(a, b, 0, self.vx, self.vs_x, half, g.dx, 1, s))
+        t(events, "y_drift", lambda: self._adv(b, a, 1, self.vy, self.vs_y, half, g.dy, 1, s))
         if self.physics_mode:
-            run("rho", lambda: check(lib().vp_row_sum_f64(a.data_ptr(), Nx * Ny, Nvx * Nvy, g.dvx, g.dvy, 0.0,
-                                                          self.rho.data_ptr(), s), "vp_row_sum_f64"))
-            run("poisson", lambda: check(lib().vp_poisson_2d_f64(
+            t(events, "rho", lambda: check(lib().vp_row_sum_f64(a.data_ptr(), Nx * Ny, Nvx * Nvy, g.dvx, g.dvy, 0.0,
+                                                                self.rho.data_ptr(), s), "vp_row_sum_f64"))
+            t(events, "poisson", lambda: check(lib().vp_poisson_2d_f64(
                 self.rho.data_ptr(), Nx, Ny, self.kx.data_ptr(), self.ky.data_ptr(), self.iks.data_ptr(),
                 Ex.data_ptr(), Ey.data_ptr(), drho.data_ptr(), phi.data_ptr(), self.pws.data_ptr(),
                 self.pws.numel(), s), "vp_poisson_2d_f64"))
-        run("vx_kick", lambda: self._adv(a, b, 2, Ex, self.vs_e, self.dt, g.dvx, 0, s))
-        run("vy_kick", lambda: self._adv(b, a, 3, Ey, self.vs_e, self.dt, g.dvy, 0, s))
-        run("y_drift2", lambda: self._adv(a, b, 1, self.vy, self.vs_y, half, g.dy, 1, s))
-        run("x_drift2", lambda: self._adv(b, a, 0, self.vx, self.vs_x, half, g.dx, 1, s))
-        # result is back in `a`
+        else:
+            self.fields.zero_()
+        if energy is not None:
+            t(events, "field_energy", lambda: check(lib().vp_field_energy_f64(
+                Ex.data_ptr(), Ey.data_ptr(), Nx * Ny, g.dx, g.dy, energy.data_ptr(), s), "field_energy"))
+        t(events, "vx_kick", lambda: self._adv(a, b, 2, Ex, self.vs_e, self.dt, g.dvx, 0, s))
+        t(events, "vy_kick", lambda: self._adv(b, a, 3, Ey, self.vs_e, self.dt, g.dvy, 0, s))
+        t(events, "y_drift2", lambda: self._adv(a, b, 1, self.vy, self.vs_y, half, g.dy, 1, s))
+        t(events, "x_drift2", lambda: self._adv(b, a, 0, self.vx, self.vs_x, half, g.dx, 1, s))
