@@ -113,7 +113,7 @@ int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, 
 
 /* vlasov_step_impl (P2:943-998): X, Y (dt/2), Poisson, Vx, Vy (dt), Y, X (dt/2).
  * Ex, Ey, delta_rho, phi: [nx][ny] outputs; zero when physics_mode == 0.
- * For nx, ny, nvx, nvy in {64, 128} the step runs the fused in-place kernels
+ * For nx, ny, nvx, nvy in {32, 64, 128} the step runs the fused in-place kernels
  * (X drift, Y drift + rho, Vx+Vy kick per (x,y) plane); other shapes run the
  * generic sweeps.  Both give identical f. */
 size_t vp_step_2d2v_workspace(const int64_t shape[4]);
@@ -176,10 +176,21 @@ enum {
     VP_KNOB_X_SPLIT = 0,       /* y-range splits of the X drift grid            */
     VP_KNOB_Y_SPLIT = 1,       /* vx-range splits of the Y drift grid           */
     VP_KNOB_KICK_PREFETCH = 2, /* 1: prefetch the next (vx,vy) plane in the kick */
-    VP_KNOB_COUNT = 3
+    VP_KNOB_DRIFT_DEPTH = 3,   /* drift tiles in flight: 1 or 2 (0 = 2)          */
+    VP_KNOB_COUNT = 4
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);
+
+/* ---------------------------------------------------------------------
+ * HBM calibration (STREAM-style; the reference ships the same kind of
+ * benchmark, simulations/stream/pytorch/src/stream.py:92-105).
+ * kind 0: copy a -> b; 1: read a (b receives `grid` partial sums);
+ * 2: write b.  n doubles (even), 16-byte aligned pointers, unroll in
+ * {1,2,4,8}, grid = number of 256-thread workgroups.
+ * --------------------------------------------------------------------- */
+int vp_stream_f64(int kind, const double* a, double* b, int64_t n, int unroll, int nontemporal,
+                  int grid, vp_stream_t stream);
 
 #ifdef __cplusplus
 }

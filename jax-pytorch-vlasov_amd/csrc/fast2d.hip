@@ -62,7 +62,8 @@ __device__ __forceinline__ double periodic_tap_sum(const double* tile, int il, c
 //   a  = fixed outer index (X: vx, Y: x),  q = vy chunk,  s = split of the loop
 //   element (row r, column c, loop l) = fin[a*a_stride + q*32 + c + r*row_stride + l*loop_stride]
 // LANE_W: the weight column is the vy index (Y drift) instead of a (X drift).
-template <int R, int PASSES, bool RHO, bool LANE_W>
+// D: tiles kept in flight (register prefetch depth, 1 or 2).
+template <int R, int PASSES, bool RHO, bool LANE_W, int D>
 __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
     const double* fin, double* fout, int nq, int S, int64_t a_stride, int64_t row_stride,
     int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
@@ -91,19 +92,27 @@ __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
     }
     const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / S);
     const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / S);
-    double cur[R];
+    double cur[R], nxt[R];
     KSum acc[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         acc[i] = KSum{0.0, 0.0};
         cur[i] = (l0 < l1) ? fin[base + (g + kGroups * i) * row_stride + l0 * loop_stride] : 0.0;
+        if (D == 2) nxt[i] = (l0 + 1 < l1) ? fin[base + (g + kGroups * i) * row_stride + (l0 + 1) * loop_stride] : 0.0;
     }
     for (int l = l0; l < l1; ++l) {
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < R; ++i) tile[(g + kGroups * i) * kCol + c] = cur[i];
         __syncthreads();
-        if (l + 1 < l1) {
+        if (D == 2) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) cur[i] = nxt[i];
+            if (l + 2 < l1) {
+#pragma unroll
+                for (int i = 0; i < R; ++i) nxt[i] = fin[base + (g + kGroups * i) * row_stride + (l + 2) * loop_stride];
+            }
+        } else if (l + 1 < l1) {
 #pragma unroll
             for (int i = 0; i < R; ++i) cur[i] = fin[base + (g + kGroups * i) * row_stride + (l + 1) * loop_stride];
         }
@@ -262,7 +271,7 @@ __global__ __launch_bounds__(1024) void kick_kernel(const double* fin, double* f
 // Host side
 // ---------------------------------------------------------------------------
 // Launch-shape knobs (vp_tune_set); 0 = automatic choice.
-static int g_knobs[VP_KNOB_COUNT] = {0, 0, 1};
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 1, 0};
 
 int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
 
@@ -345,9 +354,18 @@ int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const
     const int64_t blocks = nvx * nq * S;
     const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
     const int nvel = static_cast<int>(nvx), lt = static_cast<int>(ny);
-#define VP_X(RV, PV)                                                                                             \
-    hipLaunchKernelGGL((drift_kernel<RV, PV, false, false>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, \
-                       nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel, nullptr)
+    const int depth = knob(VP_KNOB_DRIFT_DEPTH) == 1 ? 1 : 2;
+#define VP_X(RV, PV)                                                                                           \
+    do {                                                                                                       \
+        if (depth == 2)                                                                                        \
+            hipLaunchKernelGGL((drift_kernel<RV, PV, false, false, 2>), dim3(blocks), dim3(kDriftThreads), 0, \
+                               s, fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel,  \
+                               nullptr);                                                                       \
+        else                                                                                                   \
+            hipLaunchKernelGGL((drift_kernel<RV, PV, false, false, 1>), dim3(blocks), dim3(kDriftThreads), 0, \
+                               s, fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel,  \
+                               nullptr);                                                                       \
+    } while (0)
     if (nx == 128) {
         if (passes == 2) VP_X(4, 2); else VP_X(4, 1);
     } else if (nx == 64) {
@@ -371,9 +389,18 @@ int fast2d_ydrift(const double* fin, double* fout, const int64_t shape[4], const
     const int64_t blocks = nx * nq * S;
     const int64_t a_stride = ny * nvx * nvy, row_stride = nvx * nvy, loop_stride = nvy;
     const int nvel = static_cast<int>(nvy), lt = static_cast<int>(nvx);
-#define VP_Y(RV, RH)                                                                                              \
-    hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, nq, S, \
-                       a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel, L.partial)
+    const int depth = knob(VP_KNOB_DRIFT_DEPTH) == 1 ? 1 : 2;
+#define VP_Y(RV, RH)                                                                                         \
+    do {                                                                                                     \
+        if (depth == 2 && !(RH))                                                                             \
+            hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true, 2>), dim3(blocks), dim3(kDriftThreads), 0, s, \
+                               fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel,   \
+                               L.partial);                                                                   \
+        else                                                                                                 \
+            hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true, 1>), dim3(blocks), dim3(kDriftThreads), 0, s, \
+                               fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel,   \
+                               L.partial);                                                                   \
+    } while (0)
     if (ny == 128) {
         if (rho) VP_Y(4, true); else VP_Y(4, false);
     } else if (ny == 64) {

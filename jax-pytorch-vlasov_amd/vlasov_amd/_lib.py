@@ -54,9 +54,10 @@ _SIGS = {
     "vp_field_energy_f64": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
     "vp_tune_set": (c_int, [c_int, c_int]),
     "vp_tune_get": (c_int, [c_int]),
+    "vp_stream_f64": (c_int, [c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_vp]),
 }
 
-KNOBS = {"x_split": 0, "y_split": 1, "kick_prefetch": 2}
+KNOBS = {"x_split": 0, "y_split": 1, "kick_prefetch": 2, "drift_depth": 3}
 
 _lib = None
 _lock = threading.Lock()

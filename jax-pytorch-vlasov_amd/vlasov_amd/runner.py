@@ -3,7 +3,7 @@
 ``Run2D`` owns the state f, the field arrays and the workspace and advances
 them step after step without allocating.  Two kernel paths:
 
-* fused (nx, ny, nvx, nvy in {64, 128}): in-place kernels, one C-ABI call
+* fused (nx, ny, nvx, nvy in {32, 64, 128}): in-place kernels, one C-ABI call
   per kernel (so a caller can bracket any of them with events), the closing
   X(dt/2) of a step and the opening X(dt/2) of the next run as one pass;
 * generic (any shape): the six separate sweeps of the reference step,
@@ -24,7 +24,8 @@ GENERIC_KERNELS = ("x_drift", "y_drift", "rho", "poisson", "vx_kick", "vy_kick",
 
 
 def fused_supported(shape) -> bool:
-    return all(int(n) in (64, 128) for n in shape)
+    """True when the library has fused kernels for this f shape."""
+    return len(shape) == 4 and lib().vp_run_2d2v_workspace(_lib.i64x4(shape)) > 0
 
 
 class Run2D:
@@ -39,7 +40,7 @@ class Run2D:
         self.shape = tuple(int(n) for n in f.shape)
         self.fused = fused_supported(self.shape) if fused is None else bool(fused)
         if self.fused and not fused_supported(self.shape):
-            raise ValueError(f"the fused path needs every extent in {{64, 128}}, got {self.shape}")
+            raise ValueError(f"no fused kernels for shape {self.shape} (every extent must be 32, 64 or 128)")
         Nx, Ny = self.shape[0], self.shape[1]
         self.f = [f.contiguous().clone()]
         if not self.fused:
