@@ -138,13 +138,16 @@ int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, double dt, co
                     size_t ws_bytes, vp_stream_t stream);
 
 /* Kernel-level pieces of vp_run_2d2v_f64 / the fused vp_step_2d2v_f64, in
- * the order they run (ws laid out as vp_run_2d2v_workspace); callers that
- * time or interleave the kernels issue them one by one:
+ * the order they run (ws laid out as vp_run_2d2v_workspace: weight tables,
+ * rho partial sums and one f-sized buffer); callers that time or interleave
+ * the kernels issue them one by one:
  *   tables(dt/2)                                      once per (grid, dt)
  *   xdrift(passes=1)                                  X(dt/2)
- *   per step: ydrift(with_rho=1), poisson, kick(dt), ydrift(0),
- *             xdrift(passes=2 between steps, 1 after the last)
- * xdrift/ydrift/kick may run in place (f_in == f_out). */
+ *   per step: ydrift(with_rho=1), poisson, kick(dt) into a second buffer,
+ *             ydrift(0) back, xdrift(passes=2 between steps, 1 after the last)
+ * xdrift/ydrift may run in place (f_in == f_out).  kick streams the plane
+ * rows when f_in and f_out do not overlap; in place it stages each whole
+ * (vx, vy) plane in LDS instead (slower). */
 int vp_fused_tables_f64(const int64_t shape[4], const double* vx, const double* vy, double half_dt,
                         double dx, double dy, void* ws, size_t ws_bytes, vp_stream_t stream);
 int vp_fused_xdrift_f64(const double* f_in, double* f_out, const int64_t shape[4], int passes,
@@ -177,7 +180,8 @@ enum {
     VP_KNOB_Y_SPLIT = 1,       /* vx-range splits of the Y drift grid           */
     VP_KNOB_KICK_PREFETCH = 2, /* 1: prefetch the next (vx,vy) plane in the kick */
     VP_KNOB_DRIFT_DEPTH = 3,   /* drift tiles in flight: 1 or 2 (0 = 2)          */
-    VP_KNOB_COUNT = 4
+    VP_KNOB_KICK_KERNEL = 4,   /* 1: whole-plane LDS kick even out of place      */
+    VP_KNOB_COUNT = 5
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

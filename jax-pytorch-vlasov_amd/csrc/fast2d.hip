@@ -267,11 +267,119 @@ __global__ __launch_bounds__(1024) void kick_kernel(const double* fin, double* f
     }
 }
 
+// Row-streaming Vx+Vy kick (out of place: fin and fout must not overlap).
+// One lane per vy column; a (vx, vy) plane's rows are streamed in order.
+// The Vx kick of row r reads input rows il(r)-2 .. il(r)+3 of the lane's
+// column; when il(r) - r is the same for every row ("regular" plane, the
+// normal case) those six values live in a register window refilled by one
+// load per row, issued PF rows ahead; otherwise each tap is read from fin.
+// The Vx result of row r goes through a 2-row LDS buffer to the Vy kick,
+// which is stored right away.  Small LDS footprint -> many planes per CU.
+template <int NVX, int NVY, int PF>
+__global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restrict__ fin, double* __restrict__ fout,
+                                                          int nplanes, const double* __restrict__ Ex,
+                                                          const double* __restrict__ Ey, double dt, double dvx,
+                                                          double dvy) {
+    constexpr int PPW = 128 / NVY;  // planes per workgroup
+    static_assert(NVX % PF == 0, "NVX must be a multiple of PF");
+    __shared__ double wx[PPW][NVX * 6];
+    __shared__ int ilx[PPW][NVX];
+    __shared__ double rowbuf[2][PPW * NVY];
+    __shared__ int irregular[PPW];
+    const int t = threadIdx.x;
+    const int pg = t / NVY;
+    const int c = t % NVY;
+    const int p_raw = blockIdx.x * PPW + pg;
+    const bool live = p_raw < nplanes;
+    const int p = live ? p_raw : nplanes - 1;  // lanes of a missing plane still join every barrier
+
+    if (c == 0) irregular[pg] = 0;
+    const double ex = Ex[p];
+    for (int r = c; r < NVX; r += NVY) {
+        const Departure d = departure(r, ex, dt, dvx);
+        double w[6];
+        lagrange5_weights<double>(d.alpha, w);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wx[pg][r * 6 + k] = w[k];
+        ilx[pg][r] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
+    }
+    double wy[6], my[6];
+    int jy[6];
+    {
+        const Departure d = departure(c, Ey[p], dt, dvy);
+        lagrange5_weights<double>(d.alpha, wy);
+        const int l = static_cast<int>(d.il < -8 ? -8 : (d.il > NVY + 8 ? NVY + 8 : d.il));
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int j = l + k - 2;
+            jy[k] = j < 0 ? 0 : (j > NVY - 1 ? NVY - 1 : j);
+            my[k] = (j >= 0 && j < NVY) ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    const int s0 = ilx[pg][0];
+    for (int r = c; r < NVX; r += NVY)
+        if (ilx[pg][r] - r != s0) irregular[pg] = 1;
+    __syncthreads();
+    const bool regular = irregular[pg] == 0;
+
+    const double* src = fin + static_cast<int64_t>(p) * NVX * NVY + c;
+    double* dst = fout + static_cast<int64_t>(p) * NVX * NVY + c;
+    double* rb0 = &rowbuf[0][pg * NVY];
+    double* rb1 = &rowbuf[1][pg * NVY];
+    const double* wrow = wx[pg];
+    auto clampr = [](int j) { return j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j); };
+    auto maskr = [](int j) { return (j >= 0 && j < NVX) ? 1.0 : 0.0; };
+
+    double win[5], q[PF];
+    if (regular) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) win[k] = src[clampr(s0 - 2 + k) * NVY] * maskr(s0 - 2 + k);
+#pragma unroll
+        for (int u = 0; u < PF; ++u) q[u] = src[clampr(s0 + 3 + u) * NVY];
+    }
+    for (int r0 = 0; r0 < NVX; r0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int r = r0 + u;
+            const double* w = wrow + r * 6;
+            double ox;
+            if (regular) {
+                const int jn = r + s0 + 3;  // row held by q[u]
+                const double v5 = q[u] * maskr(jn);
+                ox = w[0] * win[0] + w[1] * win[1] + w[2] * win[2] + w[3] * win[3] + w[4] * win[4] + w[5] * v5;
+                win[0] = win[1];
+                win[1] = win[2];
+                win[2] = win[3];
+                win[3] = win[4];
+                win[4] = v5;
+                q[u] = src[clampr(jn + PF) * NVY];
+            } else {
+                const int l = ilx[pg][r];
+                double v[6], wl[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    wl[k] = w[k];
+                    v[k] = src[clampr(l + k - 2) * NVY] * maskr(l + k - 2);
+                }
+                ox = tap_sum<double>(wl, v);
+            }
+            double* rb = (r & 1) ? rb1 : rb0;
+            rb[c] = ox;
+            __syncthreads();
+            double v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = rb[jy[k]] * my[k];
+            if (live) dst[r * NVY] = tap_sum<double>(wy, v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 // Launch-shape knobs (vp_tune_set); 0 = automatic choice.
-static int g_knobs[VP_KNOB_COUNT] = {0, 0, 1, 0};
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 1, 0, 0};
 
 int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
 
@@ -323,6 +431,7 @@ Fast2DWs fast2d_layout(const int64_t shape[4], void* ws) {
     L.ILy = reinterpret_cast<int*>(take(static_cast<size_t>(ny) * nvy * sizeof(int)));
     L.nparts = static_cast<int>((nvy / kCol) * y_split(nx, nvx, nvy));
     L.partial = reinterpret_cast<double2*>(take(static_cast<size_t>(nx) * ny * L.nparts * sizeof(double2)));
+    L.ftmp = reinterpret_cast<double*>(take(static_cast<size_t>(nx) * ny * nvx * nvy * sizeof(double)));
     L.bytes = off;
     return L;
 }
@@ -436,10 +545,37 @@ static int launch_kick(const double* fin, double* fout, int nplanes, const doubl
     return VP_OK;
 }
 
+template <int NVX, int NVY>
+static int launch_kick_stream(const double* fin, double* fout, int nplanes, const double* Ex, const double* Ey,
+                              double dt, double dvx, double dvy, hipStream_t s) {
+    constexpr int PPW = 128 / NVY;
+    const int grid = (nplanes + PPW - 1) / PPW;
+    hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, 8>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, Ey, dt,
+                       dvx, dvy);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
 int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const double* Ex, const double* Ey,
                 double dt, double dvx, double dvy, hipStream_t s) {
     const int64_t nvx = shape[2], nvy = shape[3];
     const int nplanes = static_cast<int>(shape[0] * shape[1]);
+    const int64_t cells = shape[0] * shape[1] * nvx * nvy;
+    const bool disjoint = fout + cells <= fin || fin + cells <= fout;
+    if (disjoint && knob(VP_KNOB_KICK_KERNEL) != 1) {
+#define VP_KS(A, B) \
+    if (nvx == A && nvy == B) return launch_kick_stream<A, B>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+        VP_KS(128, 128)
+        VP_KS(128, 64)
+        VP_KS(128, 32)
+        VP_KS(64, 128)
+        VP_KS(64, 64)
+        VP_KS(64, 32)
+        VP_KS(32, 128)
+        VP_KS(32, 64)
+        VP_KS(32, 32)
+#undef VP_KS
+    }
     const bool pf = knob(VP_KNOB_KICK_PREFETCH) != 0;
 #define VP_K(A, B)                                                                                  \
     if (nvx == A && nvy == B)                                                                       \

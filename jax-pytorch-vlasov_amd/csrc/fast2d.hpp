@@ -13,6 +13,7 @@ struct Fast2DWs {
     int* ILy;
     double2* partial;
     int nparts;
+    double* ftmp;  // one f-sized buffer (the out-of-place kick writes here)
     size_t bytes;
 };
 

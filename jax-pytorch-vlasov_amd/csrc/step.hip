@@ -88,7 +88,7 @@ extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, i
 // ---------------------------------------------------------------------------
 // 2D2V: vlasov_step_impl (vlasov2D2V.py:943-998)
 // generic ws: [N^4] f scratch | [nx*ny] rho | poisson ws
-// fused ws:   weight tables | rho partials | poisson ws   (sweeps run in place)
+// fused ws:   weight tables | rho partials | f-sized kick buffer | poisson ws
 // ---------------------------------------------------------------------------
 static size_t generic_2d_ws(const int64_t shape[4]) {
     const size_t cells = static_cast<size_t>(shape[0]) * shape[1] * shape[2] * shape[3];
@@ -137,8 +137,8 @@ static int fused_step_2d(const double* f_in, double* f_out, const int64_t shape[
         VP_TRY(vp::fill(delta_rho, nx * ny, 0.0, s));
         VP_TRY(vp::fill(phi, nx * ny, 0.0, s));
     }
-    VP_TRY(vp::fast2d_kick(f_out, f_out, shape, Ex, Ey, dt, dvx, dvy, s));
-    VP_TRY(vp::fast2d_ydrift(f_out, f_out, shape, L, false, s));
+    VP_TRY(vp::fast2d_kick(f_out, L.ftmp, shape, Ex, Ey, dt, dvx, dvy, s));
+    VP_TRY(vp::fast2d_ydrift(L.ftmp, f_out, shape, L, false, s));
     VP_TRY(vp::fast2d_xdrift(f_out, f_out, shape, L, 1, s));
     return VP_OK;
 }
@@ -225,8 +225,8 @@ extern "C" int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, do
             VP_TRY(vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, nx, ny, kx, ky, inv_ksq, Ex, Ey,
                                       delta_rho, phi, pws, vp::poisson2d_ws(nx, ny), s));
         if (energy) VP_TRY(vp::field_energy(Ex, Ey, nx * ny, dx, dy, energy + st, s));
-        VP_TRY(vp::fast2d_kick(f, f, shape, Ex, Ey, dt, dvx, dvy, s));
-        VP_TRY(vp::fast2d_ydrift(f, f, shape, L, false, s));
+        VP_TRY(vp::fast2d_kick(f, L.ftmp, shape, Ex, Ey, dt, dvx, dvy, s));
+        VP_TRY(vp::fast2d_ydrift(L.ftmp, f, shape, L, false, s));
         VP_TRY(vp::fast2d_xdrift(f, f, shape, L, st == nsteps - 1 ? 1 : 2, s));
     }
     return VP_OK;

@@ -3,9 +3,10 @@
 ``Run2D`` owns the state f, the field arrays and the workspace and advances
 them step after step without allocating.  Two kernel paths:
 
-* fused (nx, ny, nvx, nvy in {32, 64, 128}): in-place kernels, one C-ABI call
-  per kernel (so a caller can bracket any of them with events), the closing
-  X(dt/2) of a step and the opening X(dt/2) of the next run as one pass;
+* fused (nx, ny, nvx, nvy in {32, 64, 128}): one C-ABI call per kernel (so a
+  caller can bracket any of them with events), drifts in place, the kick out
+  of place into a second f-sized buffer, the closing X(dt/2) of a step and
+  the opening X(dt/2) of the next run as one pass;
 * generic (any shape): the six separate sweeps of the reference step,
   ping-ponging between two copies of f.
 
@@ -42,9 +43,7 @@ class Run2D:
         if self.fused and not fused_supported(self.shape):
             raise ValueError(f"no fused kernels for shape {self.shape} (every extent must be 32, 64 or 128)")
         Nx, Ny = self.shape[0], self.shape[1]
-        self.f = [f.contiguous().clone()]
-        if not self.fused:
-            self.f.append(torch.empty_like(self.f[0]))
+        self.f = [f.contiguous().clone(), torch.empty_like(f)]
         self.fields = torch.zeros((4, Nx, Ny), dtype=torch.float64, device=dev)  # Ex, Ey, delta_rho, phi
         f64 = torch.float64
         self.vx = grid.vx.to(f64).contiguous()
@@ -115,8 +114,8 @@ class Run2D:
         g = self.g
         s = stream_handle(self.device)
         L = lib()
-        f = self.f[0]
-        fp = f.data_ptr()
+        fp = self.f[0].data_ptr()
+        fp2 = self.f[1].data_ptr()  # the kick writes here; the closing Y drift reads it back
         ws, wsn = self.ws.data_ptr(), self.ws.numel()
         Ex, Ey, drho, phi = (t.data_ptr() for t in self.fields)
         sh = self.c_shape
@@ -139,9 +138,9 @@ class Run2D:
                 ep = energy.data_ptr() + 8 * st
                 t(events, "field_energy", lambda: check(L.vp_field_energy_f64(
                     Ex, Ey, self.shape[0] * self.shape[1], g.dx, g.dy, ep, s), "field_energy"))
-            t(events, "kick", lambda: check(L.vp_fused_kick_f64(fp, fp, sh, Ex, Ey, self.dt, g.dvx, g.dvy, s),
+            t(events, "kick", lambda: check(L.vp_fused_kick_f64(fp, fp2, sh, Ex, Ey, self.dt, g.dvx, g.dvy, s),
                                             "kick"))
-            t(events, "y_drift", lambda: check(L.vp_fused_ydrift_f64(fp, fp, sh, 0, ws, wsn, s), "ydrift"))
+            t(events, "y_drift", lambda: check(L.vp_fused_ydrift_f64(fp2, fp, sh, 0, ws, wsn, s), "ydrift"))
             last = st == nsteps - 1
             t(events, "x_drift" if last else "xx_drift",
               lambda: check(L.vp_fused_xdrift_f64(fp, fp, sh, 1 if last else 2, ws, wsn, s), "xdrift"))
