@@ -178,10 +178,11 @@ int vp_field_energy_f64(const double* Ex, const double* Ey, int64_t n, double dx
 enum {
     VP_KNOB_X_SPLIT = 0,       /* y-range splits of the X drift grid            */
     VP_KNOB_Y_SPLIT = 1,       /* vx-range splits of the Y drift grid           */
-    VP_KNOB_KICK_PREFETCH = 2, /* 1: prefetch the next (vx,vy) plane in the kick */
-    VP_KNOB_DRIFT_DEPTH = 3,   /* drift tiles in flight: 1 or 2 (0 = 2)          */
-    VP_KNOB_KICK_KERNEL = 4,   /* 1: whole-plane LDS kick even out of place      */
-    VP_KNOB_COUNT = 5
+    VP_KNOB_KICK_PREFETCH = 2, /* row-streaming kick: 2 = 16 rows ahead, else 8;
+                                  in-place plane kick: != 0 prefetches the next
+                                  plane (default 2)                            */
+    VP_KNOB_KICK_KERNEL = 3,   /* 1: whole-plane LDS kick even out of place      */
+    VP_KNOB_COUNT = 4
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

@@ -62,8 +62,7 @@ __device__ __forceinline__ double periodic_tap_sum(const double* tile, int il, c
 //   a  = fixed outer index (X: vx, Y: x),  q = vy chunk,  s = split of the loop
 //   element (row r, column c, loop l) = fin[a*a_stride + q*32 + c + r*row_stride + l*loop_stride]
 // LANE_W: the weight column is the vy index (Y drift) instead of a (X drift).
-// D: tiles kept in flight (register prefetch depth, 1 or 2).
-template <int R, int PASSES, bool RHO, bool LANE_W, int D>
+template <int R, int PASSES, bool RHO, bool LANE_W>
 __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
     const double* fin, double* fout, int nq, int S, int64_t a_stride, int64_t row_stride,
     int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
@@ -90,32 +89,35 @@ __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
         for (int k = 0; k < 6; ++k) w[i][k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
         il[i] = IL[static_cast<int64_t>(r) * nvel + j];
     }
-    const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / S);
-    const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / S);
-    double cur[R], nxt[R];
-    KSum acc[R];
+    // Consume the weight loads here: otherwise the wait-count analysis of the
+    // loop body also waits for the in-flight tile prefetch before each FMA.
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        acc[i] = KSum{0.0, 0.0};
-        cur[i] = (l0 < l1) ? fin[base + (g + kGroups * i) * row_stride + l0 * loop_stride] : 0.0;
-        if (D == 2) nxt[i] = (l0 + 1 < l1) ? fin[base + (g + kGroups * i) * row_stride + (l0 + 1) * loop_stride] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w[i][k]));
+        asm volatile("" ::"v"(il[i]));
     }
-    for (int l = l0; l < l1; ++l) {
-        __syncthreads();
+    const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / S);
+    const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / S);
+    // Software pipeline: tile l is in LDS while tile l+1 is in flight into
+    // `cur`.  The wait for `cur` comes after the stores of tile l, on the
+    // first iteration as on every later one, so the wait-count analysis waits
+    // for the loads only (stores may still be draining).
+    double cur[R];
+    KSum acc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) acc[i] = KSum{0.0, 0.0};
+    if (l0 < l1) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) cur[i] = fin[base + (g + kGroups * i) * row_stride + l0 * loop_stride];
 #pragma unroll
         for (int i = 0; i < R; ++i) tile[(g + kGroups * i) * kCol + c] = cur[i];
         __syncthreads();
-        if (D == 2) {
+        const int ln = (l0 + 1 < l1) ? l0 + 1 : l0;
 #pragma unroll
-            for (int i = 0; i < R; ++i) cur[i] = nxt[i];
-            if (l + 2 < l1) {
-#pragma unroll
-                for (int i = 0; i < R; ++i) nxt[i] = fin[base + (g + kGroups * i) * row_stride + (l + 2) * loop_stride];
-            }
-        } else if (l + 1 < l1) {
-#pragma unroll
-            for (int i = 0; i < R; ++i) cur[i] = fin[base + (g + kGroups * i) * row_stride + (l + 1) * loop_stride];
-        }
+        for (int i = 0; i < R; ++i) cur[i] = fin[base + (g + kGroups * i) * row_stride + ln * loop_stride];
+    }
+    for (int l = l0; l < l1; ++l) {
         if (PASSES == 2) {
             double out[R];
 #pragma unroll
@@ -131,6 +133,13 @@ __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
             fout[base + (g + kGroups * i) * row_stride + l * loop_stride] = o;
             if (RHO) ksum_add(acc[i], o);
         }
+        __syncthreads();  // every read of tile l done
+#pragma unroll
+        for (int i = 0; i < R; ++i) tile[(g + kGroups * i) * kCol + c] = cur[i];
+        __syncthreads();
+        const int ln = (l + 2 < l1) ? l + 2 : l1 - 1;  // the last iterations re-read a tile they own
+#pragma unroll
+        for (int i = 0; i < R; ++i) cur[i] = fin[base + (g + kGroups * i) * row_stride + ln * loop_stride];
     }
     if (RHO) {
         const int nparts = nq * S;
@@ -285,15 +294,13 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
     __shared__ double wx[PPW][NVX * 6];
     __shared__ int ilx[PPW][NVX];
     __shared__ double rowbuf[2][PPW * NVY];
-    __shared__ int irregular[PPW];
+    __shared__ int irregular[1];
     const int t = threadIdx.x;
     const int pg = t / NVY;
     const int c = t % NVY;
-    const int p_raw = blockIdx.x * PPW + pg;
-    const bool live = p_raw < nplanes;
-    const int p = live ? p_raw : nplanes - 1;  // lanes of a missing plane still join every barrier
+    const int p = blockIdx.x * PPW + pg;  // the host launches exactly nplanes / PPW workgroups
 
-    if (c == 0) irregular[pg] = 0;
+    if (t == 0) irregular[0] = 0;
     const double ex = Ex[p];
     for (int r = c; r < NVX; r += NVY) {
         const Departure d = departure(r, ex, dt, dvx);
@@ -319,9 +326,9 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
     __syncthreads();
     const int s0 = ilx[pg][0];
     for (int r = c; r < NVX; r += NVY)
-        if (ilx[pg][r] - r != s0) irregular[pg] = 1;
+        if (ilx[pg][r] - r != s0) irregular[0] = 1;  // one flag for the whole workgroup
     __syncthreads();
-    const bool regular = irregular[pg] == 0;
+    const bool regular = irregular[0] == 0;  // uniform: every plane of the group takes one path
 
     const double* src = fin + static_cast<int64_t>(p) * NVX * NVY + c;
     double* dst = fout + static_cast<int64_t>(p) * NVX * NVY + c;
@@ -330,47 +337,52 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
     const double* wrow = wx[pg];
     auto clampr = [](int j) { return j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j); };
     auto maskr = [](int j) { return (j >= 0 && j < NVX) ? 1.0 : 0.0; };
+    // Vy kick of row r from the Vx result ox (through the 2-row LDS buffer).
+    auto vy_row = [&](int r, double ox) {
+        double* rb = (r & 1) ? rb1 : rb0;
+        rb[c] = ox;
+        __syncthreads();
+        double v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = rb[jy[k]] * my[k];
+        dst[r * NVY] = tap_sum<double>(wy, v);
+    };
 
-    double win[5], q[PF];
     if (regular) {
+        double win[5], q[PF];
 #pragma unroll
         for (int k = 0; k < 5; ++k) win[k] = src[clampr(s0 - 2 + k) * NVY] * maskr(s0 - 2 + k);
 #pragma unroll
         for (int u = 0; u < PF; ++u) q[u] = src[clampr(s0 + 3 + u) * NVY];
-    }
-    for (int r0 = 0; r0 < NVX; r0 += PF) {
+        for (int r0 = 0; r0 < NVX; r0 += PF) {
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int r = r0 + u;
-            const double* w = wrow + r * 6;
-            double ox;
-            if (regular) {
+            for (int u = 0; u < PF; ++u) {
+                const int r = r0 + u;
+                const double* w = wrow + r * 6;
                 const int jn = r + s0 + 3;  // row held by q[u]
                 const double v5 = q[u] * maskr(jn);
-                ox = w[0] * win[0] + w[1] * win[1] + w[2] * win[2] + w[3] * win[3] + w[4] * win[4] + w[5] * v5;
+                const double ox =
+                    w[0] * win[0] + w[1] * win[1] + w[2] * win[2] + w[3] * win[3] + w[4] * win[4] + w[5] * v5;
                 win[0] = win[1];
                 win[1] = win[2];
                 win[2] = win[3];
                 win[3] = win[4];
                 win[4] = v5;
                 q[u] = src[clampr(jn + PF) * NVY];
-            } else {
-                const int l = ilx[pg][r];
-                double v[6], wl[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    wl[k] = w[k];
-                    v[k] = src[clampr(l + k - 2) * NVY] * maskr(l + k - 2);
-                }
-                ox = tap_sum<double>(wl, v);
+                vy_row(r, ox);
             }
-            double* rb = (r & 1) ? rb1 : rb0;
-            rb[c] = ox;
-            __syncthreads();
-            double v[6];
+        }
+    } else {
+        for (int r = 0; r < NVX; ++r) {
+            const double* w = wrow + r * 6;
+            const int l = ilx[pg][r];
+            double v[6], wl[6];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) v[k] = rb[jy[k]] * my[k];
-            if (live) dst[r * NVY] = tap_sum<double>(wy, v);
+            for (int k = 0; k < 6; ++k) {
+                wl[k] = w[k];
+                v[k] = src[clampr(l + k - 2) * NVY] * maskr(l + k - 2);
+            }
+            vy_row(r, tap_sum<double>(wl, v));
         }
     }
 }
@@ -379,7 +391,7 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
 // Host side
 // ---------------------------------------------------------------------------
 // Launch-shape knobs (vp_tune_set); 0 = automatic choice.
-static int g_knobs[VP_KNOB_COUNT] = {0, 0, 1, 0, 0};
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0};
 
 int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
 
@@ -393,6 +405,11 @@ static int cu_count() {
         cached[dev] = v;
     }
     return cached[dev];
+}
+
+bool fast2d_kick_supported(const int64_t shape[4]) {
+    auto ok = [](int64_t n) { return n == 32 || n == 64 || n == 128; };
+    return ok(shape[2]) && ok(shape[3]);
 }
 
 bool fast2d_supported(const int64_t shape[4]) {
@@ -463,18 +480,9 @@ int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const
     const int64_t blocks = nvx * nq * S;
     const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
     const int nvel = static_cast<int>(nvx), lt = static_cast<int>(ny);
-    const int depth = knob(VP_KNOB_DRIFT_DEPTH) == 1 ? 1 : 2;
-#define VP_X(RV, PV)                                                                                           \
-    do {                                                                                                       \
-        if (depth == 2)                                                                                        \
-            hipLaunchKernelGGL((drift_kernel<RV, PV, false, false, 2>), dim3(blocks), dim3(kDriftThreads), 0, \
-                               s, fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel,  \
-                               nullptr);                                                                       \
-        else                                                                                                   \
-            hipLaunchKernelGGL((drift_kernel<RV, PV, false, false, 1>), dim3(blocks), dim3(kDriftThreads), 0, \
-                               s, fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel,  \
-                               nullptr);                                                                       \
-    } while (0)
+#define VP_X(RV, PV)                                                                                            \
+    hipLaunchKernelGGL((drift_kernel<RV, PV, false, false>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, \
+                       nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel, nullptr)
     if (nx == 128) {
         if (passes == 2) VP_X(4, 2); else VP_X(4, 1);
     } else if (nx == 64) {
@@ -498,18 +506,9 @@ int fast2d_ydrift(const double* fin, double* fout, const int64_t shape[4], const
     const int64_t blocks = nx * nq * S;
     const int64_t a_stride = ny * nvx * nvy, row_stride = nvx * nvy, loop_stride = nvy;
     const int nvel = static_cast<int>(nvy), lt = static_cast<int>(nvx);
-    const int depth = knob(VP_KNOB_DRIFT_DEPTH) == 1 ? 1 : 2;
-#define VP_Y(RV, RH)                                                                                         \
-    do {                                                                                                     \
-        if (depth == 2 && !(RH))                                                                             \
-            hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true, 2>), dim3(blocks), dim3(kDriftThreads), 0, s, \
-                               fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel,   \
-                               L.partial);                                                                   \
-        else                                                                                                 \
-            hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true, 1>), dim3(blocks), dim3(kDriftThreads), 0, s, \
-                               fin, fout, nq, S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel,   \
-                               L.partial);                                                                   \
-    } while (0)
+#define VP_Y(RV, RH)                                                                                          \
+    hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, nq, \
+                       S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel, L.partial)
     if (ny == 128) {
         if (rho) VP_Y(4, true); else VP_Y(4, false);
     } else if (ny == 64) {
@@ -549,9 +548,14 @@ template <int NVX, int NVY>
 static int launch_kick_stream(const double* fin, double* fout, int nplanes, const double* Ex, const double* Ey,
                               double dt, double dvx, double dvy, hipStream_t s) {
     constexpr int PPW = 128 / NVY;
-    const int grid = (nplanes + PPW - 1) / PPW;
-    hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, 8>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, Ey, dt,
-                       dvx, dvy);
+    if (nplanes % PPW != 0) return VP_ERR_UNSUPPORTED;
+    const int grid = nplanes / PPW;
+    if (knob(VP_KNOB_KICK_PREFETCH) == 2)
+        hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, 16>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, Ey,
+                           dt, dvx, dvy);
+    else
+        hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, 8>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, Ey,
+                           dt, dvx, dvy);
     VP_CHECK_LAUNCH();
     return VP_OK;
 }
@@ -563,8 +567,11 @@ int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const d
     const int64_t cells = shape[0] * shape[1] * nvx * nvy;
     const bool disjoint = fout + cells <= fin || fin + cells <= fout;
     if (disjoint && knob(VP_KNOB_KICK_KERNEL) != 1) {
-#define VP_KS(A, B) \
-    if (nvx == A && nvy == B) return launch_kick_stream<A, B>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+#define VP_KS(A, B)                                                                                  \
+    if (nvx == A && nvy == B) {                                                                      \
+        const int rc = launch_kick_stream<A, B>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);        \
+        if (rc != VP_ERR_UNSUPPORTED) return rc;                                                     \
+    }
         VP_KS(128, 128)
         VP_KS(128, 64)
         VP_KS(128, 32)

@@ -18,6 +18,8 @@ struct Fast2DWs {
 };
 
 bool fast2d_supported(const int64_t shape[4]);
+// the kick only constrains the (vx, vy) plane
+bool fast2d_kick_supported(const int64_t shape[4]);
 // Carves the workspace (ws == nullptr: only computes .bytes).
 Fast2DWs fast2d_layout(const int64_t shape[4], void* ws);
 int fast2d_tables(const int64_t shape[4], const Fast2DWs& L, const double* vx, const double* vy, double half_dt,

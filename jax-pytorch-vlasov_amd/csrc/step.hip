@@ -188,7 +188,8 @@ extern "C" int vp_fused_poisson_f64(const int64_t shape[4], double dvx, double d
 extern "C" int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_t shape[4], const double* Ex,
                                  const double* Ey, double dt, double dvx, double dvy, vp_stream_t stream) {
     VP_RETURN_IF(!shape || !f_in || !f_out || !Ex || !Ey, VP_ERR_ARG);
-    VP_RETURN_IF(!vp::fast2d_supported(shape), VP_ERR_UNSUPPORTED);
+    VP_RETURN_IF(shape[0] < 1 || shape[1] < 1 || shape[0] * shape[1] > 0x7fffffffLL, VP_ERR_SHAPE);
+    VP_RETURN_IF(!vp::fast2d_kick_supported(shape), VP_ERR_UNSUPPORTED);
     return vp::fast2d_kick(f_in, f_out, shape, Ex, Ey, dt, dvx, dvy, static_cast<hipStream_t>(stream));
 }
 
