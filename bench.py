@@ -38,7 +38,58 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N, help="points per axis (default 128)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
     return ap.parse_args()
+
+
+def stream_copy_gbs(n: int, reps: int = 10) -> float:
+    """Measured HBM copy bandwidth (GB/s, read + write bytes) of the library's
+    STREAM copy kernel on two float64 arrays of n^4 elements, in its fastest
+    measured form: one 16-byte element per thread, one workgroup per 256
+    elements (no grid-stride loop), non-temporal loads and stores."""
+    import torch
+
+    from vlasov_amd import _lib
+
+    lib = _lib.lib()
+    cells = n ** 4
+    a = torch.ones(cells, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    s = torch.cuda.current_stream().cuda_stream
+    grid = cells // 2 // 256
+
+    def copy():
+        _lib.check(lib.vp_stream_f64(0, a.data_ptr(), b.data_ptr(), cells, 1, 1, grid, s), "vp_stream_f64")
+
+    copy()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        copy()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 8 * cells / (ms * 1e-3) / 1e9
+
+
+def traffic_from_profile(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` at n^4 from the committed counter
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when no summary covers this kernel and size."""
+    path = os.path.join(ROOT, "profiles", "traffic_2d2v_fp64.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if data.get("n") != n:
+        return None
+    k = data.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes")
 
 
 def cpu_baseline(n: int) -> dict:
@@ -112,6 +163,10 @@ def main():
     n = args.n
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    copy_gbs = stream_copy_gbs(n)
+    if args.stream_only:
+        print(json.dumps({"stream_copy_GBps": copy_gbs, "bytes_per_launch": 2 * 8 * n ** 4}), flush=True)
+        return
     grid = v2.Grid.create(n, n, n, n, L, L, VMAX, VMAX, device=dev, dtype=torch.float64)
     f = configs.two_stream_2d(grid)
     run = runner.Run2D(grid, f, DT, physics_mode=True)
@@ -161,8 +216,9 @@ def main():
         "kernel_calls": calls,
         "field_energy_last": float(energy[args.steps - 1].item()),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(dom, n),
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "measured_copy_GBps": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)

@@ -182,7 +182,14 @@ enum {
                                   in-place plane kick: != 0 prefetches the next
                                   plane (default 2)                            */
     VP_KNOB_KICK_KERNEL = 3,   /* 1: whole-plane LDS kick even out of place      */
-    VP_KNOB_COUNT = 4
+    VP_KNOB_DRIFT_DMA = 4,     /* drifts at 64/128 rows: 1 = register prefetch,
+                                  2 or 3 = direct-to-LDS ring of that many
+                                  tiles (0 = 2)                                */
+    VP_KNOB_NONTEMPORAL = 5,   /* sweeps whose f loads/stores carry the
+                                  streaming (non-temporal) hint, a bit mask:
+                                  1 X drift, 2 Y drift, 4 kick; 8 = none
+                                  (0 = 7)                                      */
+    VP_KNOB_COUNT = 6
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

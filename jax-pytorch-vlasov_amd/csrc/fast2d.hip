@@ -160,6 +160,175 @@ __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
     }
 }
 
+// Wait until at most N vector memory operations of this wave are outstanding.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// Direct-to-LDS load of 16 bytes per lane: lane i's bytes land at LDS byte
+// address lds + 16 * i.  Written as inline asm so that the compiler does not
+// treat the pending LDS write as a hazard for every later LDS read (it would
+// otherwise wait for all such loads, including the prefetches, before each
+// read); the caller orders it with wait_vmcnt and a barrier.
+template <bool NT>
+__device__ __forceinline__ void load_lds_x4(const void* gp, uint32_t lds) {
+    if (NT)
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off nt" ::"s"(lds), "v"(gp) : "memory", "m0");
+    else
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(gp) : "memory", "m0");
+}
+
+// Store with the non-temporal (streaming) hint when NT: f is far larger than
+// every cache level, so nothing written by a sweep is re-read from cache.
+template <bool NT>
+__device__ __forceinline__ void store_f(double* p, double v) {
+    if (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t lds_addr(T* p) {
+    typedef __attribute__((address_space(3))) T* LdsPtr;
+    return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((LdsPtr)(p)));
+}
+
+// Same sweep as drift_kernel, but the tiles stream into an LDS ring of T
+// slots with direct-to-LDS loads (global_load_lds_dwordx4: each lane moves 16
+// bytes, a wave fills 1 KB = 4 tile rows), so T - 1 tiles are in flight per
+// workgroup without holding registers; the registers keep the weights.
+// Rows: n in {64, 128} (P = n / 64 load instructions per wave per tile).
+// Per iteration i (tile l = l0 + i): wait until this wave's loads of tile l
+// have landed, barrier, start the load of tile l + T - 1 into the slot freed
+// by tile l - 1, compute and store tile l (R stores).  The loads and stores
+// share the in-order vmcnt counter, so after the load of tile l this wave
+// has issued exactly (T - 2) * P + min(i, T - 1) * R further vector memory
+// operations; waiting for vmcnt <= that number waits for tile l and nothing
+// younger.  For T > 2 the tail iterations re-load the last tile (an L2 hit)
+// so the count stays the same to the end; for T = 2 the only load a wait
+// depends on is the one issued in the iteration before, and the last
+// iteration issues none.
+template <int R, int PASSES, bool RHO, bool LANE_W, int T, bool NT>
+__global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
+    const double* fin, double* fout, int nq, int S, int64_t a_stride, int64_t row_stride,
+    int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
+    double2* __restrict__ partial) {
+    constexpr int n = R * kGroups;
+    constexpr int P = n / 64;
+    static_assert(P * 64 == n && P >= 1, "rows must be a multiple of 64");
+    static_assert(T >= 2 && T <= 4, "ring of 2..4 tiles");  // the waits below cover T <= 4
+    __shared__ __align__(16) double ring[T][n * kCol];
+    const int t = threadIdx.x;
+    const int c = t & (kCol - 1);
+    const int g = t / kCol;
+    const int wave = t >> 6;
+    const int lane = t & 63;
+    int b = blockIdx.x;
+    const int s = b % S;
+    b /= S;
+    const int q = b % nq;
+    const int a = b / nq;
+    const int64_t base = static_cast<int64_t>(a) * a_stride + q * kCol + c;
+    const int j = LANE_W ? q * kCol + c : a;
+
+    double w[R][6];
+    int il[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = g + kGroups * i;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[i][k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
+        il[i] = IL[static_cast<int64_t>(r) * nvel + j];
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w[i][k]));
+        asm volatile("" ::"v"(il[i]));
+    }
+    const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / S);
+    const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / S);
+    if (l0 >= l1) return;  // uniform over the workgroup
+
+    // Load instruction k of this wave covers tile rows 4m .. 4m+3, m = wave + 16k;
+    // lane -> row 4m + lane/16, columns 2*(lane%16) .. +1.
+    const double* src_lane[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int row = 4 * (wave + 16 * k) + (lane >> 4);
+        src_lane[k] = fin + static_cast<int64_t>(a) * a_stride + q * kCol + 2 * (lane & 15) + row * row_stride;
+    }
+    uint32_t dst_wave[T];
+#pragma unroll
+    for (int u = 0; u < T; ++u) dst_wave[u] = __builtin_amdgcn_readfirstlane(lds_addr(&ring[u][4 * wave * kCol]));
+    auto issue = [&](int tile_l, int slot) {
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            load_lds_x4<NT>(src_lane[k] + static_cast<int64_t>(tile_l) * loop_stride,
+                        dst_wave[slot] + k * (16 * 4 * kCol * static_cast<uint32_t>(sizeof(double))));
+    };
+#pragma unroll
+    for (int u = 0; u < T - 1; ++u) issue(l0 + u < l1 ? l0 + u : l1 - 1, u);
+
+    KSum acc[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) acc[i] = KSum{0.0, 0.0};
+    for (int l = l0; l < l1; ++l) {
+        const int i = l - l0;
+        const int slot = i % T;
+        if (i == 0) wait_vmcnt<(T - 2) * P>();
+        else if (T > 2 && i == 1) wait_vmcnt<(T - 2) * P + R>();
+        else if (T > 3 && i == 2) wait_vmcnt<(T - 2) * P + 2 * R>();
+        else wait_vmcnt<(T - 2) * P + (T - 1) * R>();
+        __syncthreads();
+        if (T == 2) {
+            // with two slots nothing waits for a load issued in the last
+            // iteration, so the tail issues none
+            if (l + 1 < l1) issue(l + 1, (i + 1) % T);
+        } else {
+            const int ln = l + T - 1 < l1 ? l + T - 1 : l1 - 1;
+            issue(ln, (i + T - 1) % T);
+        }
+        double* tile = ring[slot];
+        if (PASSES == 2) {
+            double out[R];
+#pragma unroll
+            for (int ii = 0; ii < R; ++ii) out[ii] = periodic_tap_sum<R>(tile, il[ii], w[ii], n, c);
+            __syncthreads();
+#pragma unroll
+            for (int ii = 0; ii < R; ++ii) tile[(g + kGroups * ii) * kCol + c] = out[ii];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int ii = 0; ii < R; ++ii) {
+            const double o = periodic_tap_sum<R>(tile, il[ii], w[ii], n, c);
+            store_f<NT>(&fout[base + (g + kGroups * ii) * row_stride + l * loop_stride], o);
+            if (RHO) ksum_add(acc[ii], o);
+        }
+    }
+    wait_vmcnt<0>();  // drain the tail re-loads before the workgroup's LDS is released
+    if (RHO) {
+        const int nparts = nq * S;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            KSum k = acc[i];
+            for (int off = kCol / 2; off > 0; off >>= 1) {
+                KSum o;
+                o.s = __shfl_xor(k.s, off, kCol);
+                o.c = __shfl_xor(k.c, off, kCol);
+                ksum_merge(k, o);
+            }
+            if (c == 0) {
+                const int r = g + kGroups * i;
+                partial[(static_cast<int64_t>(a) * n + r) * nparts + q * S + s] = make_double2(k.s, k.c);
+            }
+        }
+    }
+}
+
 // Dirichlet tap (clamp + zero mask, vlasov2D2V.py:340-345).
 __device__ __forceinline__ double dirichlet_tap(const double* p, int j, int n, int stride) {
     const int jc = j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
@@ -284,7 +453,7 @@ __global__ __launch_bounds__(1024) void kick_kernel(const double* fin, double* f
 // load per row, issued PF rows ahead; otherwise each tap is read from fin.
 // The Vx result of row r goes through a 2-row LDS buffer to the Vy kick,
 // which is stored right away.  Small LDS footprint -> many planes per CU.
-template <int NVX, int NVY, int PF>
+template <int NVX, int NVY, int PF, bool NT>
 __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restrict__ fin, double* __restrict__ fout,
                                                           int nplanes, const double* __restrict__ Ex,
                                                           const double* __restrict__ Ey, double dt, double dvx,
@@ -345,15 +514,16 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
         double v[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = rb[jy[k]] * my[k];
-        dst[r * NVY] = tap_sum<double>(wy, v);
+        store_f<NT>(&dst[r * NVY], tap_sum<double>(wy, v));
     };
+    auto ld = [](const double* p) { return NT ? __builtin_nontemporal_load(p) : *p; };
 
     if (regular) {
         double win[5], q[PF];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) win[k] = src[clampr(s0 - 2 + k) * NVY] * maskr(s0 - 2 + k);
+        for (int k = 0; k < 5; ++k) win[k] = ld(&src[clampr(s0 - 2 + k) * NVY]) * maskr(s0 - 2 + k);
 #pragma unroll
-        for (int u = 0; u < PF; ++u) q[u] = src[clampr(s0 + 3 + u) * NVY];
+        for (int u = 0; u < PF; ++u) q[u] = ld(&src[clampr(s0 + 3 + u) * NVY]);
         for (int r0 = 0; r0 < NVX; r0 += PF) {
 #pragma unroll
             for (int u = 0; u < PF; ++u) {
@@ -368,7 +538,7 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
                 win[2] = win[3];
                 win[3] = win[4];
                 win[4] = v5;
-                q[u] = src[clampr(jn + PF) * NVY];
+                q[u] = ld(&src[clampr(jn + PF) * NVY]);
                 vy_row(r, ox);
             }
         }
@@ -380,7 +550,7 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
                 wl[k] = w[k];
-                v[k] = src[clampr(l + k - 2) * NVY] * maskr(l + k - 2);
+                v[k] = ld(&src[clampr(l + k - 2) * NVY]) * maskr(l + k - 2);
             }
             vy_row(r, tap_sum<double>(wl, v));
         }
@@ -391,9 +561,15 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
 // Host side
 // ---------------------------------------------------------------------------
 // Launch-shape knobs (vp_tune_set); 0 = automatic choice.
-static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0};
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0, 0, 0};
 
 int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
+
+// Sweeps that use the streaming hint: 1 = X drift, 2 = Y drift, 4 = kick.
+static int nt_mask() {
+    const int k = knob(VP_KNOB_NONTEMPORAL);
+    return k == 0 ? 7 : (k & 7);
+}
 
 static int cu_count() {
     static int cached[64] = {0};
@@ -423,10 +599,12 @@ static inline size_t a256(size_t b) { return (b + 255) & ~static_cast<size_t>(25
 static int y_split(int64_t nx, int64_t nvx, int64_t nvy) {
     int S = knob(VP_KNOB_Y_SPLIT);
     if (S <= 0) {
-        // enough workgroups to cover the chip at one 1024-thread workgroup per CU
+        // About 24 workgroups per CU over the launch (one 1024-thread
+        // workgroup is resident per CU), at least 4 tiles per workgroup.
+        // Measured at 128^4: 12 splits (10-11 tiles each) beat 1, 8, 16, 32.
         const int64_t base = nx * (nvy / kCol);
-        S = 1;
-        while (base * S < 2 * 256 && S < 8) S *= 2;
+        const int64_t want = (24 * static_cast<int64_t>(cu_count()) + base / 2) / base;
+        S = static_cast<int>(want < 1 ? 1 : (want > nvx / 4 ? nvx / 4 : want));
     }
     if (S > nvx) S = static_cast<int>(nvx);
     return S;
@@ -466,6 +644,34 @@ int fast2d_tables(const int64_t shape[4], const Fast2DWs& L, const double* vx, c
     return VP_OK;
 }
 
+// Drift launch: the LDS-ring kernel (T = 2 or 3 tiles, default 2) for 64 and
+// 128 rows unless the DRIFT_DMA knob is 1 or fin is not 16-byte aligned, else
+// the register-prefetch kernel.
+template <int R, int PASSES, bool RHO, bool LANE_W>
+static void launch_drift(const double* fin, double* fout, int64_t blocks, int nq, int S, int64_t a_stride,
+                         int64_t row_stride, int64_t loop_stride, int lt, const double* W, const int* IL, int nvel,
+                         double2* partial, hipStream_t s) {
+    const int mode = knob(VP_KNOB_DRIFT_DMA);
+    const bool nt = (nt_mask() & (LANE_W ? 2 : 1)) != 0;
+    const bool aligned = (reinterpret_cast<uintptr_t>(fin) & 15) == 0;
+    if constexpr (R >= 2) {
+        if (mode != 1 && aligned) {
+#define VP_D(T, NT)                                                                                                   \
+    hipLaunchKernelGGL((drift_dma_kernel<R, PASSES, RHO, LANE_W, T, NT>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, \
+                       fout, nq, S, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial)
+            if (mode == 3) {
+                if (nt) VP_D(3, true); else VP_D(3, false);
+            } else {
+                if (nt) VP_D(2, true); else VP_D(2, false);
+            }
+#undef VP_D
+            return;
+        }
+    }
+    hipLaunchKernelGGL((drift_kernel<R, PASSES, RHO, LANE_W>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, nq,
+                       S, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial);
+}
+
 // X drift (passes = 1 or 2) from fin to fout (may alias).
 int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, int passes,
                   hipStream_t s) {
@@ -474,15 +680,15 @@ int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const
     int S = knob(VP_KNOB_X_SPLIT);
     if (S <= 0) {
         S = 1;
-        while (nvx * nq * S < 4 * 256 && S < static_cast<int>(ny)) S *= 2;
+        while (nvx * nq * S < 2 * cu_count() && S < static_cast<int>(ny)) S *= 2;
     }
     if (S > static_cast<int>(ny)) S = static_cast<int>(ny);
     const int64_t blocks = nvx * nq * S;
     const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
     const int nvel = static_cast<int>(nvx), lt = static_cast<int>(ny);
-#define VP_X(RV, PV)                                                                                            \
-    hipLaunchKernelGGL((drift_kernel<RV, PV, false, false>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, \
-                       nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, nvel, nullptr)
+#define VP_X(RV, PV)                                                                                          \
+    launch_drift<RV, PV, false, false>(fin, fout, blocks, nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, \
+                                       nvel, nullptr, s)
     if (nx == 128) {
         if (passes == 2) VP_X(4, 2); else VP_X(4, 1);
     } else if (nx == 64) {
@@ -506,9 +712,9 @@ int fast2d_ydrift(const double* fin, double* fout, const int64_t shape[4], const
     const int64_t blocks = nx * nq * S;
     const int64_t a_stride = ny * nvx * nvy, row_stride = nvx * nvy, loop_stride = nvy;
     const int nvel = static_cast<int>(nvy), lt = static_cast<int>(nvx);
-#define VP_Y(RV, RH)                                                                                          \
-    hipLaunchKernelGGL((drift_kernel<RV, 1, RH, true>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, nq, \
-                       S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel, L.partial)
+#define VP_Y(RV, RH)                                                                                      \
+    launch_drift<RV, 1, RH, true>(fin, fout, blocks, nq, S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, \
+                                  nvel, L.partial, s)
     if (ny == 128) {
         if (rho) VP_Y(4, true); else VP_Y(4, false);
     } else if (ny == 64) {
@@ -550,12 +756,17 @@ static int launch_kick_stream(const double* fin, double* fout, int nplanes, cons
     constexpr int PPW = 128 / NVY;
     if (nplanes % PPW != 0) return VP_ERR_UNSUPPORTED;
     const int grid = nplanes / PPW;
-    if (knob(VP_KNOB_KICK_PREFETCH) == 2)
-        hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, 16>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, Ey,
-                           dt, dvx, dvy);
-    else
-        hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, 8>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, Ey,
-                           dt, dvx, dvy);
+    const bool pf16 = knob(VP_KNOB_KICK_PREFETCH) == 2;
+    const bool nt = (nt_mask() & 4) != 0;
+#define VP_K(PF, NT)                                                                                           \
+    hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, PF, NT>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, \
+                       Ey, dt, dvx, dvy)
+    if (pf16) {
+        if (nt) VP_K(16, true); else VP_K(16, false);
+    } else {
+        if (nt) VP_K(8, true); else VP_K(8, false);
+    }
+#undef VP_K
     VP_CHECK_LAUNCH();
     return VP_OK;
 }

@@ -57,7 +57,7 @@ _SIGS = {
     "vp_stream_f64": (c_int, [c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_vp]),
 }
 
-KNOBS = {"x_split": 0, "y_split": 1, "kick_prefetch": 2, "kick_kernel": 3}
+KNOBS = {"x_split": 0, "y_split": 1, "kick_prefetch": 2, "kick_kernel": 3, "drift_dma": 4, "nontemporal": 5}
 
 _lib = None
 _lock = threading.Lock()

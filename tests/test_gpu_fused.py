@@ -110,7 +110,9 @@ def test_fused_xdrift_two_passes_equal_two_calls(device):
 
 
 @pytest.mark.parametrize("knobs", [{"x_split": 1, "y_split": 1, "kick_prefetch": 0},
-                                   {"x_split": 16, "y_split": 4, "kick_prefetch": 1}])
+                                   {"x_split": 16, "y_split": 4, "kick_prefetch": 1},
+                                   {"drift_dma": 1}, {"drift_dma": 2, "nontemporal": 8},
+                                   {"drift_dma": 3, "x_split": 3, "kick_prefetch": 1, "nontemporal": 7}])
 def test_launch_knobs_do_not_change_results(device, knobs):
     v2, runner, L = _mods()
     lib = L.lib()
