@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N, help="points per axis (default 128)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage comparison")
+    ap.add_argument("--fp32-error-steps", type=int, default=300,
+                    help="steps of the FP32-storage vs float64 error comparison (default 300)")
     ap.add_argument("--stream-only", action="store_true",
                     help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
     return ap.parse_args()
@@ -74,6 +77,64 @@ def stream_copy_gbs(n: int, reps: int = 10) -> float:
     del a, b
     torch.cuda.empty_cache()
     return 2 * 8 * cells / (ms * 1e-3) / 1e9
+
+
+def fp32_storage(grid, f0, steps: int, warmup: int, err_steps: int) -> dict:
+    """FP32-storage variant on the same workload: speed (wall time per step,
+    no per-kernel events) and the error of f, phi, Ex, Ey and the
+    field-energy history after `err_steps` steps against the float64 path
+    (which matches the float64 oracle to <= 1e-12 in the parity tests)."""
+    import torch
+
+    from vlasov_amd import runner
+
+    out = {}
+    for dtype in (torch.float64, torch.float32):
+        run = runner.Run2D(grid, f0.to(dtype), DT, physics_mode=True)
+        run.advance(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.advance(steps)
+        torch.cuda.synchronize()
+        out[dtype] = (time.perf_counter() - t0) / steps
+        del run
+        torch.cuda.empty_cache()
+    hist = {}
+    for dtype in (torch.float64, torch.float32):
+        run = runner.Run2D(grid, f0.to(dtype), DT, physics_mode=True)
+        energy = torch.zeros(err_steps, dtype=torch.float64, device=f0.device)
+        run.advance(err_steps, energy=energy)
+        torch.cuda.synchronize()
+        hist[dtype] = (run.state, run.phi.clone(), run.Ex.clone(), run.Ey.clone(), energy)
+    a, b = hist[torch.float64], hist[torch.float32]
+
+    def rel(x, y):
+        return float((y.double() - x).abs().max() / x.abs().max())
+
+    w64, w32 = a[4], b[4]
+    res = {
+        "ms_per_step": out[torch.float32] * 1e3,
+        "value": grid.Nx * grid.Ny * grid.Nvx * grid.Nvy / out[torch.float32],
+        "fp64_ms_per_step_same_timing": out[torch.float64] * 1e3,
+        "speedup_vs_fp64": out[torch.float64] / out[torch.float32],
+        "error_steps": err_steps,
+        "error_reference": "float64 path of this library, same initial f, same steps",
+        "max_rel_err_f": rel(a[0], b[0]),
+        "max_rel_err_phi": rel(a[1], b[1]),
+        "max_rel_err_Ex": rel(a[2], b[2]),
+        "max_rel_err_Ey": rel(a[3], b[3]),
+        "max_rel_err_field_energy": float(((w32 - w64).abs() / w64.abs()).max()),
+        "max_err_field_energy_over_max": float((w32 - w64).abs().max() / w64.abs().max()),
+        "steps_within_1e-5_field_energy": _steps_within(((w32 - w64).abs() / w64.abs()), 1e-5),
+        "field_energy_last_fp64": float(w64[-1]),
+    }
+    return res
+
+
+def _steps_within(rel_err, tol: float) -> int:
+    """Number of leading steps whose relative error stays <= tol."""
+    bad = (rel_err > tol).nonzero()
+    return int(bad[0, 0]) if bad.numel() else int(rel_err.numel())
 
 
 def traffic_from_profile(kernel: str, n: int):
@@ -168,9 +229,8 @@ def main():
         print(json.dumps({"stream_copy_GBps": copy_gbs, "bytes_per_launch": 2 * 8 * n ** 4}), flush=True)
         return
     grid = v2.Grid.create(n, n, n, n, L, L, VMAX, VMAX, device=dev, dtype=torch.float64)
-    f = configs.two_stream_2d(grid)
-    run = runner.Run2D(grid, f, DT, physics_mode=True)
-    del f
+    f0 = configs.two_stream_2d(grid)
+    run = runner.Run2D(grid, f0, DT, physics_mode=True)
     run.advance(args.warmup)
     torch.cuda.synchronize()
 
@@ -220,6 +280,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "measured_copy_GBps": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs},
     }
+    del run
+    torch.cuda.empty_cache()
+    if not args.no_fp32 and runner.fused_supported(f0.shape, torch.float32):
+        out["fp32_storage"] = fp32_storage(grid, f0, args.steps, args.warmup, args.fp32_error_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

@@ -138,9 +138,10 @@ int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, double dt, co
                     size_t ws_bytes, vp_stream_t stream);
 
 /* Kernel-level pieces of vp_run_2d2v_f64 / the fused vp_step_2d2v_f64, in
- * the order they run (ws laid out as vp_run_2d2v_workspace: weight tables,
- * rho partial sums and one f-sized buffer); callers that time or interleave
- * the kernels issue them one by one:
+ * the order they run (ws laid out as vp_fused_workspace(shape, 8): weight
+ * tables, rho partial sums, Poisson scratch; the caller owns the second
+ * f-sized buffer the kick writes); callers that time or interleave the
+ * kernels issue them one by one:
  *   tables(dt/2)                                      once per (grid, dt)
  *   xdrift(passes=1)                                  X(dt/2)
  *   per step: ydrift(with_rho=1), poisson, kick(dt) into a second buffer,
@@ -148,6 +149,7 @@ int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, double dt, co
  * xdrift/ydrift may run in place (f_in == f_out).  kick streams the plane
  * rows when f_in and f_out do not overlap; in place it stages each whole
  * (vx, vy) plane in LDS instead (slower). */
+size_t vp_fused_workspace(const int64_t shape[4], int elem_bytes); /* 8: f64 pieces, 4: f32; 0 = none */
 int vp_fused_tables_f64(const int64_t shape[4], const double* vx, const double* vy, double half_dt,
                         double dx, double dy, void* ws, size_t ws_bytes, vp_stream_t stream);
 int vp_fused_xdrift_f64(const double* f_in, double* f_out, const int64_t shape[4], int passes,
@@ -159,6 +161,36 @@ int vp_fused_poisson_f64(const int64_t shape[4], double dvx, double dvy, const d
                          double* delta_rho, double* phi, void* ws, size_t ws_bytes,
                          vp_stream_t stream);
 int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_t shape[4], const double* Ex,
+                      const double* Ey, double dt, double dvx, double dvy, vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * FP32-storage variant of the fused run (no reference counterpart: the
+ * reference's float32 mode computes everything in float32).  f is float32
+ * in HBM and interpolated in float32 with weights computed in float64 and
+ * rounded once; the charge density is summed in float64 from the stored
+ * values, and the Poisson solve, fields and field energy stay float64.
+ * Same arguments and step order as vp_run_2d2v_f64.  Shapes: nx, ny, nvy in
+ * {64, 128}, nvx in {32, 64, 128} (vp_run_2d2v_f32_workspace returns 0
+ * otherwise).  The pieces take ws laid out as vp_fused_workspace(shape, 4);
+ * the kick must run out of place (f_in and f_out disjoint).
+ * --------------------------------------------------------------------- */
+size_t vp_run_2d2v_f32_workspace(const int64_t shape[4]);
+int vp_run_2d2v_f32(float* f, const int64_t shape[4], int nsteps, double dt, const double* vx,
+                    const double* vy, const double* kx, const double* ky, const double* inv_ksq,
+                    double dx, double dy, double dvx, double dvy, int physics_mode, double* Ex,
+                    double* Ey, double* delta_rho, double* phi, double* energy, void* ws,
+                    size_t ws_bytes, vp_stream_t stream);
+int vp_fused_tables_f32(const int64_t shape[4], const double* vx, const double* vy, double half_dt,
+                        double dx, double dy, void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_fused_xdrift_f32(const float* f_in, float* f_out, const int64_t shape[4], int passes,
+                        void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_fused_ydrift_f32(const float* f_in, float* f_out, const int64_t shape[4], int with_rho,
+                        void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_fused_poisson_f32(const int64_t shape[4], double dvx, double dvy, const double* kx,
+                         const double* ky, const double* inv_ksq, double* Ex, double* Ey,
+                         double* delta_rho, double* phi, void* ws, size_t ws_bytes,
+                         vp_stream_t stream);
+int vp_fused_kick_f32(const float* f_in, float* f_out, const int64_t shape[4], const double* Ex,
                       const double* Ey, double dt, double dvx, double dvy, vp_stream_t stream);
 
 /* ---------------------------------------------------------------------

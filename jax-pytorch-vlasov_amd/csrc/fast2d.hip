@@ -154,7 +154,10 @@ __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
             }
             if (c == 0) {
                 const int r = g + kGroups * i;
-                partial[(static_cast<int64_t>(a) * n + r) * nparts + q * S + s] = make_double2(k.s, k.c);
+                // part-major layout [part][x * ny + y]: the Poisson kernel reads it coalesced
+                const int64_t npoints = static_cast<int64_t>(gridDim.x / nparts) * n;
+                partial[static_cast<int64_t>(q * S + s) * npoints + static_cast<int64_t>(a) * n + r] =
+                    make_double2(k.s, k.c);
             }
         }
     }
@@ -174,16 +177,18 @@ __device__ __forceinline__ void wait_vmcnt() {
 // read); the caller orders it with wait_vmcnt and a barrier.
 template <bool NT>
 __device__ __forceinline__ void load_lds_x4(const void* gp, uint32_t lds) {
+    // M0 holds the wave's LDS base address; binding it as an operand lets the
+    // compiler set it (and know it is live) instead of the asm clobbering it.
     if (NT)
-        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off nt" ::"s"(lds), "v"(gp) : "memory", "m0");
+        asm volatile("global_load_lds_dwordx4 %1, off nt" ::"{m0}"(lds), "v"(gp) : "memory");
     else
-        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(gp) : "memory", "m0");
+        asm volatile("global_load_lds_dwordx4 %1, off" ::"{m0}"(lds), "v"(gp) : "memory");
 }
 
 // Store with the non-temporal (streaming) hint when NT: f is far larger than
 // every cache level, so nothing written by a sweep is re-read from cache.
-template <bool NT>
-__device__ __forceinline__ void store_f(double* p, double v) {
+template <bool NT, typename S>
+__device__ __forceinline__ void store_f(S* p, S v) {
     if (NT)
         __builtin_nontemporal_store(v, p);
     else
@@ -196,11 +201,37 @@ __device__ __forceinline__ uint32_t lds_addr(T* p) {
     return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((LdsPtr)(p)));
 }
 
+// Tile geometry of the LDS-ring drift for storage type S: a tile row is 256
+// contiguous bytes (32 float64 or 64 float32 columns); 1024 threads.
+template <typename S>
+struct RingGeom {
+    static constexpr int col = 256 / static_cast<int>(sizeof(S));
+    static constexpr int groups = kDriftThreads / col;
+};
+
+// Six taps of one row from a tile of n (power of two) rows of 256 bytes.
+// A = byte offset of (row il - 2 mod n, this thread's column); tap k is at
+// (A + 256 k) mod 256 n: two integer operations per tap from one register per
+// row, instead of six precomputed offsets (which the compiler would otherwise
+// hoist out of the tile loop: 6 registers per row).
+template <typename S, int N>
+__device__ __forceinline__ S ring_tap_sum(const S* tile, uint32_t A, const S (&w)[6]) {
+    const char* base = reinterpret_cast<const char*>(tile);
+    S v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = *reinterpret_cast<const S*>(base + ((A + 256u * k) & (256u * N - 1)));
+    return tap_sum<S>(w, v);
+}
+
 // Same sweep as drift_kernel, but the tiles stream into an LDS ring of T
 // slots with direct-to-LDS loads (global_load_lds_dwordx4: each lane moves 16
 // bytes, a wave fills 1 KB = 4 tile rows), so T - 1 tiles are in flight per
 // workgroup without holding registers; the registers keep the weights.
-// Rows: n in {64, 128} (P = n / 64 load instructions per wave per tile).
+// S = storage (and interpolation) type of f: double, or float for the
+// FP32-storage variant (weights from the float64 table rounded to float, the
+// charge density summed in float64 from the stored float values).
+// Rows: n = R * 1024 / col in {64, 128} (P = n / 64 load instructions per wave
+// per tile).
 // Per iteration i (tile l = l0 + i): wait until this wave's loads of tile l
 // have landed, barrier, start the load of tile l + T - 1 into the slot freed
 // by tile l - 1, compute and store tile l (R stores).  The loads and stores
@@ -211,68 +242,78 @@ __device__ __forceinline__ uint32_t lds_addr(T* p) {
 // so the count stays the same to the end; for T = 2 the only load a wait
 // depends on is the one issued in the iteration before, and the last
 // iteration issues none.
-template <int R, int PASSES, bool RHO, bool LANE_W, int T, bool NT>
+template <typename S, int R, int PASSES, bool RHO, bool LANE_W, int T, bool NT>
 __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
-    const double* fin, double* fout, int nq, int S, int64_t a_stride, int64_t row_stride,
+    const S* fin, S* fout, int nq, int nsplit, int64_t a_stride, int64_t row_stride,
     int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
     double2* __restrict__ partial) {
-    constexpr int n = R * kGroups;
+    constexpr int KC = RingGeom<S>::col;
+    constexpr int KG = RingGeom<S>::groups;
+    constexpr int n = R * KG;
     constexpr int P = n / 64;
+    constexpr int PER_LANE = 16 / static_cast<int>(sizeof(S));
     static_assert(P * 64 == n && P >= 1, "rows must be a multiple of 64");
     static_assert(T >= 2 && T <= 4, "ring of 2..4 tiles");  // the waits below cover T <= 4
-    __shared__ __align__(16) double ring[T][n * kCol];
+    __shared__ __align__(16) S ring[T][n * KC];
+    __shared__ __align__(16) S mid[PASSES == 2 ? n * KC : 1];  // first-pass result of the X(dt/2) pair
     const int t = threadIdx.x;
-    const int c = t & (kCol - 1);
-    const int g = t / kCol;
+    const int c = t & (KC - 1);
+    const int g = t / KC;
     const int wave = t >> 6;
     const int lane = t & 63;
     int b = blockIdx.x;
-    const int s = b % S;
-    b /= S;
+    const int s = b % nsplit;
+    b /= nsplit;
     const int q = b % nq;
     const int a = b / nq;
-    const int64_t base = static_cast<int64_t>(a) * a_stride + q * kCol + c;
-    const int j = LANE_W ? q * kCol + c : a;
+    const int64_t base = static_cast<int64_t>(a) * a_stride + q * KC + c;
+    const int j = LANE_W ? q * KC + c : a;
 
-    double w[R][6];
-    int il[R];
+    S w[R][6];
+    uint32_t A[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        const int r = g + kGroups * i;
+        const int r = g + KG * i;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) w[i][k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
-        il[i] = IL[static_cast<int64_t>(r) * nvel + j];
+        for (int k = 0; k < 6; ++k) w[i][k] = static_cast<S>(W[(static_cast<int64_t>(r) * 6 + k) * nvel + j]);
+        const int il = IL[static_cast<int64_t>(r) * nvel + j];
+        A[i] = static_cast<uint32_t>(((il - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(S)));
     }
+    // Consume the weight loads here: otherwise the wait-count analysis of the
+    // loop body also waits for the in-flight stores before each FMA.
 #pragma unroll
     for (int i = 0; i < R; ++i) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w[i][k]));
-        asm volatile("" ::"v"(il[i]));
+        asm volatile("" ::"v"(A[i]));
     }
-    const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / S);
-    const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / S);
+    const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / nsplit);
+    const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / nsplit);
     if (l0 >= l1) return;  // uniform over the workgroup
 
     // Load instruction k of this wave covers tile rows 4m .. 4m+3, m = wave + 16k;
-    // lane -> row 4m + lane/16, columns 2*(lane%16) .. +1.
-    const double* src_lane[P];
+    // lane -> row 4m + lane/16, the 16 bytes at column PER_LANE*(lane%16).
+    const S* src_lane[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const int row = 4 * (wave + 16 * k) + (lane >> 4);
-        src_lane[k] = fin + static_cast<int64_t>(a) * a_stride + q * kCol + 2 * (lane & 15) + row * row_stride;
+        src_lane[k] = fin + static_cast<int64_t>(a) * a_stride + q * KC + PER_LANE * (lane & 15) + row * row_stride;
     }
     uint32_t dst_wave[T];
 #pragma unroll
-    for (int u = 0; u < T; ++u) dst_wave[u] = __builtin_amdgcn_readfirstlane(lds_addr(&ring[u][4 * wave * kCol]));
+    for (int u = 0; u < T; ++u) dst_wave[u] = __builtin_amdgcn_readfirstlane(lds_addr(&ring[u][4 * wave * KC]));
     auto issue = [&](int tile_l, int slot) {
 #pragma unroll
         for (int k = 0; k < P; ++k)
-            load_lds_x4<NT>(src_lane[k] + static_cast<int64_t>(tile_l) * loop_stride,
-                        dst_wave[slot] + k * (16 * 4 * kCol * static_cast<uint32_t>(sizeof(double))));
+            load_lds_x4<NT>(src_lane[k] + static_cast<int64_t>(tile_l) * loop_stride, dst_wave[slot] + k * (16 * 1024));
     };
 #pragma unroll
     for (int u = 0; u < T - 1; ++u) issue(l0 + u < l1 ? l0 + u : l1 - 1, u);
 
+    // Charge density of this thread's rows: compensated for float64 storage;
+    // for float32 storage the float64 sum of at most 128 float32 values per
+    // launch is already exact to ~1e-14, far below the storage rounding.
+    constexpr bool COMP = sizeof(S) == 8;
     KSum acc[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) acc[i] = KSum{0.0, 0.0};
@@ -292,39 +333,76 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
             const int ln = l + T - 1 < l1 ? l + T - 1 : l1 - 1;
             issue(ln, (i + T - 1) % T);
         }
-        double* tile = ring[slot];
+        const S* tile = ring[slot];
+#pragma unroll
+        for (int ii = 0; ii < R; ++ii) asm volatile("" : "+v"(A[ii]));  // keep the tap offsets in the loop
         if (PASSES == 2) {
-            double out[R];
+            // first pass into `mid` (free since the barrier above), second from it
 #pragma unroll
-            for (int ii = 0; ii < R; ++ii) out[ii] = periodic_tap_sum<R>(tile, il[ii], w[ii], n, c);
+            for (int ii = 0; ii < R; ++ii) mid[(g + KG * ii) * KC + c] = ring_tap_sum<S, n>(tile, A[ii], w[ii]);
             __syncthreads();
-#pragma unroll
-            for (int ii = 0; ii < R; ++ii) tile[(g + kGroups * ii) * kCol + c] = out[ii];
-            __syncthreads();
+            tile = mid;
         }
 #pragma unroll
         for (int ii = 0; ii < R; ++ii) {
-            const double o = periodic_tap_sum<R>(tile, il[ii], w[ii], n, c);
-            store_f<NT>(&fout[base + (g + kGroups * ii) * row_stride + l * loop_stride], o);
-            if (RHO) ksum_add(acc[ii], o);
+            const S o = ring_tap_sum<S, n>(tile, A[ii], w[ii]);
+            store_f<NT>(&fout[base + (g + KG * ii) * row_stride + l * loop_stride], o);
+            if (RHO) {
+                if (COMP)
+                    ksum_add(acc[ii], static_cast<double>(o));
+                else
+                    acc[ii].s += static_cast<double>(o);
+            }
         }
     }
-    wait_vmcnt<0>();  // drain the tail re-loads before the workgroup's LDS is released
+    wait_vmcnt<0>();  // no direct-to-LDS load may still be in flight when the workgroup ends
     if (RHO) {
-        const int nparts = nq * S;
+        // Row sums over the KC columns through LDS (the ring is free now):
+        // every thread parks its R partial sums at [row][column], then
+        // 1024 / n threads per row add KC * n / 1024 columns each and merge
+        // with a few lane shuffles.  Float64 storage keeps (sum, compensation)
+        // pairs; float32 storage plain float64 sums.
+        constexpr int TPR = kDriftThreads / n;  // threads per row
+        constexpr int CPT = KC / TPR;           // columns per thread
+        static_assert(CPT * TPR == KC && TPR <= 64, "row reduction layout");
+        constexpr int W2 = COMP ? 2 : 1;  // doubles per parked sum
+        static_assert(sizeof(ring) >= sizeof(double) * W2 * n * KC, "ring too small for the row sums");
+        double* red = reinterpret_cast<double*>(&ring[0][0]);
+        __syncthreads();  // every wave is done with the ring
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-            KSum k = acc[i];
-            for (int off = kCol / 2; off > 0; off >>= 1) {
-                KSum o;
-                o.s = __shfl_xor(k.s, off, kCol);
-                o.c = __shfl_xor(k.c, off, kCol);
+            double* e = red + ((g + KG * i) * KC + c) * W2;
+            e[0] = acc[i].s;
+            if (COMP) e[1] = acc[i].c;
+        }
+        __syncthreads();
+        const int row = t / TPR;
+        const int c0 = (t % TPR) * CPT;
+        KSum k{0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const double* e = red + (row * KC + c0 + u) * W2;
+            if (COMP)
+                ksum_merge(k, KSum{e[0], e[1]});
+            else
+                k.s += e[0];
+        }
+        for (int off = TPR / 2; off > 0; off >>= 1) {
+            KSum o;
+            o.s = __shfl_xor(k.s, off, TPR);
+            if (COMP) {
+                o.c = __shfl_xor(k.c, off, TPR);
                 ksum_merge(k, o);
+            } else {
+                k.s += o.s;
             }
-            if (c == 0) {
-                const int r = g + kGroups * i;
-                partial[(static_cast<int64_t>(a) * n + r) * nparts + q * S + s] = make_double2(k.s, k.c);
-            }
+        }
+        if (t % TPR == 0) {
+            const int nparts = nq * nsplit;
+            // part-major layout [part][x * ny + y]: the Poisson kernel reads it coalesced
+            const int64_t npoints = static_cast<int64_t>(gridDim.x / nparts) * n;
+            partial[static_cast<int64_t>(q * nsplit + s) * npoints + static_cast<int64_t>(a) * n + row] =
+                make_double2(k.s, COMP ? k.c : 0.0);
         }
     }
 }
@@ -453,16 +531,16 @@ __global__ __launch_bounds__(1024) void kick_kernel(const double* fin, double* f
 // load per row, issued PF rows ahead; otherwise each tap is read from fin.
 // The Vx result of row r goes through a 2-row LDS buffer to the Vy kick,
 // which is stored right away.  Small LDS footprint -> many planes per CU.
-template <int NVX, int NVY, int PF, bool NT>
-__global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restrict__ fin, double* __restrict__ fout,
+template <typename S, int NVX, int NVY, int PF, bool NT>
+__global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ fin, S* __restrict__ fout,
                                                           int nplanes, const double* __restrict__ Ex,
                                                           const double* __restrict__ Ey, double dt, double dvx,
                                                           double dvy) {
     constexpr int PPW = 128 / NVY;  // planes per workgroup
     static_assert(NVX % PF == 0, "NVX must be a multiple of PF");
-    __shared__ double wx[PPW][NVX * 6];
+    __shared__ S wx[PPW][NVX * 6];
     __shared__ int ilx[PPW][NVX];
-    __shared__ double rowbuf[2][PPW * NVY];
+    __shared__ S rowbuf[2][PPW * NVY];
     __shared__ int irregular[1];
     const int t = threadIdx.x;
     const int pg = t / NVY;
@@ -476,20 +554,23 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
         double w[6];
         lagrange5_weights<double>(d.alpha, w);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) wx[pg][r * 6 + k] = w[k];
+        for (int k = 0; k < 6; ++k) wx[pg][r * 6 + k] = static_cast<S>(w[k]);
         ilx[pg][r] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
     }
-    double wy[6], my[6];
+    S wy[6], my[6];
     int jy[6];
     {
         const Departure d = departure(c, Ey[p], dt, dvy);
-        lagrange5_weights<double>(d.alpha, wy);
+        double wyd[6];
+        lagrange5_weights<double>(d.alpha, wyd);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wy[k] = static_cast<S>(wyd[k]);
         const int l = static_cast<int>(d.il < -8 ? -8 : (d.il > NVY + 8 ? NVY + 8 : d.il));
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const int j = l + k - 2;
             jy[k] = j < 0 ? 0 : (j > NVY - 1 ? NVY - 1 : j);
-            my[k] = (j >= 0 && j < NVY) ? 1.0 : 0.0;
+            my[k] = (j >= 0 && j < NVY) ? S(1) : S(0);
         }
     }
     __syncthreads();
@@ -499,27 +580,27 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
     __syncthreads();
     const bool regular = irregular[0] == 0;  // uniform: every plane of the group takes one path
 
-    const double* src = fin + static_cast<int64_t>(p) * NVX * NVY + c;
-    double* dst = fout + static_cast<int64_t>(p) * NVX * NVY + c;
-    double* rb0 = &rowbuf[0][pg * NVY];
-    double* rb1 = &rowbuf[1][pg * NVY];
-    const double* wrow = wx[pg];
+    const S* src = fin + static_cast<int64_t>(p) * NVX * NVY + c;
+    S* dst = fout + static_cast<int64_t>(p) * NVX * NVY + c;
+    S* rb0 = &rowbuf[0][pg * NVY];
+    S* rb1 = &rowbuf[1][pg * NVY];
+    const S* wrow = wx[pg];
     auto clampr = [](int j) { return j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j); };
-    auto maskr = [](int j) { return (j >= 0 && j < NVX) ? 1.0 : 0.0; };
+    auto maskr = [](int j) { return (j >= 0 && j < NVX) ? S(1) : S(0); };
     // Vy kick of row r from the Vx result ox (through the 2-row LDS buffer).
-    auto vy_row = [&](int r, double ox) {
-        double* rb = (r & 1) ? rb1 : rb0;
+    auto vy_row = [&](int r, S ox) {
+        S* rb = (r & 1) ? rb1 : rb0;
         rb[c] = ox;
         __syncthreads();
-        double v[6];
+        S v[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = rb[jy[k]] * my[k];
-        store_f<NT>(&dst[r * NVY], tap_sum<double>(wy, v));
+        store_f<NT>(&dst[r * NVY], tap_sum<S>(wy, v));
     };
-    auto ld = [](const double* p) { return NT ? __builtin_nontemporal_load(p) : *p; };
+    auto ld = [](const S* p) { return NT ? __builtin_nontemporal_load(p) : *p; };
 
     if (regular) {
-        double win[5], q[PF];
+        S win[5], q[PF];
 #pragma unroll
         for (int k = 0; k < 5; ++k) win[k] = ld(&src[clampr(s0 - 2 + k) * NVY]) * maskr(s0 - 2 + k);
 #pragma unroll
@@ -528,10 +609,10 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
 #pragma unroll
             for (int u = 0; u < PF; ++u) {
                 const int r = r0 + u;
-                const double* w = wrow + r * 6;
+                const S* w = wrow + r * 6;
                 const int jn = r + s0 + 3;  // row held by q[u]
-                const double v5 = q[u] * maskr(jn);
-                const double ox =
+                const S v5 = q[u] * maskr(jn);
+                const S ox =
                     w[0] * win[0] + w[1] * win[1] + w[2] * win[2] + w[3] * win[3] + w[4] * win[4] + w[5] * v5;
                 win[0] = win[1];
                 win[1] = win[2];
@@ -544,15 +625,15 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const double* __restri
         }
     } else {
         for (int r = 0; r < NVX; ++r) {
-            const double* w = wrow + r * 6;
+            const S* w = wrow + r * 6;
             const int l = ilx[pg][r];
-            double v[6], wl[6];
+            S v[6], wl[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
                 wl[k] = w[k];
                 v[k] = ld(&src[clampr(l + k - 2) * NVY]) * maskr(l + k - 2);
             }
-            vy_row(r, tap_sum<double>(wl, v));
+            vy_row(r, tap_sum<S>(wl, v));
         }
     }
 }
@@ -594,24 +675,33 @@ bool fast2d_supported(const int64_t shape[4]) {
     return ok(nx) && ok(ny) && ok(nvx) && ok(nvy);
 }
 
+// FP32 storage: the drifts need 64 or 128 rows (LDS-ring kernel only) and
+// whole 64-column (256-byte) vy chunks.
+bool fast2d_supported_f32(const int64_t shape[4]) {
+    const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
+    auto big = [](int64_t n) { return n == 64 || n == 128; };
+    return big(nx) && big(ny) && big(nvy) && (nvx == 32 || nvx == 64 || nvx == 128);
+}
+
 static inline size_t a256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
-static int y_split(int64_t nx, int64_t nvx, int64_t nvy) {
+// Number of vx ranges the Y drift splits its loop into.
+static int y_split(int64_t nvx, int elem_bytes) {
     int S = knob(VP_KNOB_Y_SPLIT);
     if (S <= 0) {
-        // About 24 workgroups per CU over the launch (one 1024-thread
-        // workgroup is resident per CU), at least 4 tiles per workgroup.
-        // Measured at 128^4: 12 splits (10-11 tiles each) beat 1, 8, 16, 32.
-        const int64_t base = nx * (nvy / kCol);
-        const int64_t want = (24 * static_cast<int64_t>(cu_count()) + base / 2) / base;
-        S = static_cast<int>(want < 1 ? 1 : (want > nvx / 4 ? nvx / 4 : want));
+        // Measured at 128^4: float64 best at 12 splits (10-11 tiles per
+        // workgroup; 1, 8, 16, 32 slower), float32 storage at 10 (6, 8, 12, 16
+        // slower).
+        S = elem_bytes == 8 ? static_cast<int>((3 * nvx + 16) / 32) : static_cast<int>((5 * nvx + 32) / 64);
+        if (S < 1) S = 1;
     }
     if (S > nvx) S = static_cast<int>(nvx);
     return S;
 }
 
-Fast2DWs fast2d_layout(const int64_t shape[4], void* ws) {
+Fast2DWs fast2d_layout(const int64_t shape[4], void* ws, int elem_bytes, bool with_ftmp) {
     const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
+    const int col = 256 / elem_bytes;
     Fast2DWs L{};
     unsigned char* p = static_cast<unsigned char*>(ws);
     size_t off = 0;
@@ -624,9 +714,11 @@ Fast2DWs fast2d_layout(const int64_t shape[4], void* ws) {
     L.ILx = reinterpret_cast<int*>(take(static_cast<size_t>(nx) * nvx * sizeof(int)));
     L.Wy = reinterpret_cast<double*>(take(static_cast<size_t>(ny) * 6 * nvy * sizeof(double)));
     L.ILy = reinterpret_cast<int*>(take(static_cast<size_t>(ny) * nvy * sizeof(int)));
-    L.nparts = static_cast<int>((nvy / kCol) * y_split(nx, nvx, nvy));
+    L.nparts = static_cast<int>((nvy / col) * y_split(nvx, elem_bytes));
     L.partial = reinterpret_cast<double2*>(take(static_cast<size_t>(nx) * ny * L.nparts * sizeof(double2)));
-    L.ftmp = reinterpret_cast<double*>(take(static_cast<size_t>(nx) * ny * nvx * nvy * sizeof(double)));
+    L.pws_bytes = poisson2d_ws(nx, ny);
+    L.pws = take(L.pws_bytes);
+    L.ftmp = with_ftmp ? take(static_cast<size_t>(nx) * ny * nvx * nvy * elem_bytes) : nullptr;
     L.bytes = off;
     return L;
 }
@@ -644,89 +736,120 @@ int fast2d_tables(const int64_t shape[4], const Fast2DWs& L, const double* vx, c
     return VP_OK;
 }
 
-// Drift launch: the LDS-ring kernel (T = 2 or 3 tiles, default 2) for 64 and
-// 128 rows unless the DRIFT_DMA knob is 1 or fin is not 16-byte aligned, else
-// the register-prefetch kernel.
-template <int R, int PASSES, bool RHO, bool LANE_W>
-static void launch_drift(const double* fin, double* fout, int64_t blocks, int nq, int S, int64_t a_stride,
-                         int64_t row_stride, int64_t loop_stride, int lt, const double* W, const int* IL, int nvel,
-                         double2* partial, hipStream_t s) {
+// Drift launch: the LDS-ring kernel (T = 2 or 3 tiles, default 2) when it
+// covers the line length (64 or 128 rows), unless the DRIFT_DMA knob is 1 or
+// fin is not 16-byte aligned (float64 only); otherwise the register-prefetch
+// kernel (float64 only).  R = rows per thread.
+template <typename S, int R, int PASSES, bool RHO, bool LANE_W>
+static int launch_drift(const S* fin, S* fout, int64_t blocks, int nq, int nsplit, int64_t a_stride,
+                        int64_t row_stride, int64_t loop_stride, int lt, const double* W, const int* IL, int nvel,
+                        double2* partial, hipStream_t s) {
+    constexpr bool f64 = sizeof(S) == 8;
+    constexpr int rows = R * RingGeom<S>::groups;
     const int mode = knob(VP_KNOB_DRIFT_DMA);
     const bool nt = (nt_mask() & (LANE_W ? 2 : 1)) != 0;
     const bool aligned = (reinterpret_cast<uintptr_t>(fin) & 15) == 0;
-    if constexpr (R >= 2) {
-        if (mode != 1 && aligned) {
-#define VP_D(T, NT)                                                                                                   \
-    hipLaunchKernelGGL((drift_dma_kernel<R, PASSES, RHO, LANE_W, T, NT>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, \
-                       fout, nq, S, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial)
+    if constexpr (rows == 64 || rows == 128) {
+        if ((mode != 1 || !f64) && aligned) {
+#define VP_D(T, NT)                                                                                               \
+    hipLaunchKernelGGL((drift_dma_kernel<S, R, PASSES, RHO, LANE_W, T, NT>), dim3(blocks), dim3(kDriftThreads), 0, \
+                       s, fin, fout, nq, nsplit, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial)
             if (mode == 3) {
                 if (nt) VP_D(3, true); else VP_D(3, false);
             } else {
                 if (nt) VP_D(2, true); else VP_D(2, false);
             }
 #undef VP_D
-            return;
+            return VP_OK;
         }
     }
-    hipLaunchKernelGGL((drift_kernel<R, PASSES, RHO, LANE_W>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout, nq,
-                       S, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial);
+    if constexpr (f64) {
+        hipLaunchKernelGGL((drift_kernel<R, PASSES, RHO, LANE_W>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout,
+                           nq, nsplit, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial);
+        return VP_OK;
+    }
+    return VP_ERR_UNSUPPORTED;
 }
 
 // X drift (passes = 1 or 2) from fin to fout (may alias).
-int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, int passes,
-                  hipStream_t s) {
+template <typename S>
+static int xdrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs& L, int passes, hipStream_t s) {
+    constexpr int KC = RingGeom<S>::col, KG = RingGeom<S>::groups;
     const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
-    const int nq = static_cast<int>(nvy / kCol);
-    int S = knob(VP_KNOB_X_SPLIT);
-    if (S <= 0) {
-        S = 1;
-        while (nvx * nq * S < 2 * cu_count() && S < static_cast<int>(ny)) S *= 2;
+    if (nvy % KC != 0) return VP_ERR_UNSUPPORTED;
+    const int nq = static_cast<int>(nvy / KC);
+    int S_ = knob(VP_KNOB_X_SPLIT);
+    if (S_ <= 0) {
+        S_ = 1;
+        while (nvx * nq * S_ < 2 * cu_count() && S_ < static_cast<int>(ny)) S_ *= 2;
     }
-    if (S > static_cast<int>(ny)) S = static_cast<int>(ny);
-    const int64_t blocks = nvx * nq * S;
+    if (S_ > static_cast<int>(ny)) S_ = static_cast<int>(ny);
+    const int64_t blocks = nvx * nq * S_;
     const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
     const int nvel = static_cast<int>(nvx), lt = static_cast<int>(ny);
-#define VP_X(RV, PV)                                                                                          \
-    launch_drift<RV, PV, false, false>(fin, fout, blocks, nq, S, a_stride, row_stride, loop_stride, lt, L.Wx, L.ILx, \
-                                       nvel, nullptr, s)
-    if (nx == 128) {
-        if (passes == 2) VP_X(4, 2); else VP_X(4, 1);
-    } else if (nx == 64) {
-        if (passes == 2) VP_X(2, 2); else VP_X(2, 1);
-    } else if (nx == 32) {
-        if (passes == 2) VP_X(1, 2); else VP_X(1, 1);
+    int rc = VP_ERR_UNSUPPORTED;
+#define VP_X(N, PV)                                                                                                \
+    if (nx == N && (N) % KG == 0 && (N) / KG >= 1) {                                                               \
+        constexpr int RV = (N) / KG > 0 ? (N) / KG : 1;                                                            \
+        rc = launch_drift<S, RV, PV, false, false>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride, lt, \
+                                                   L.Wx, L.ILx, nvel, nullptr, s);                                 \
+    }
+    if (passes == 2) {
+        VP_X(128, 2) else VP_X(64, 2) else VP_X(32, 2)
     } else {
-        return VP_ERR_UNSUPPORTED;
+        VP_X(128, 1) else VP_X(64, 1) else VP_X(32, 1)
     }
 #undef VP_X
+    if (rc != VP_OK) return rc;
     VP_CHECK_LAUNCH();
     return VP_OK;
 }
 
 // Y drift (+ rho partials when rho != 0) from fin to fout (may alias).
-int fast2d_ydrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, bool rho,
-                  hipStream_t s) {
+template <typename S>
+static int ydrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs& L, bool rho, hipStream_t s) {
+    constexpr int KC = RingGeom<S>::col, KG = RingGeom<S>::groups;
     const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
-    const int nq = static_cast<int>(nvy / kCol);
-    const int S = y_split(nx, nvx, nvy);
-    const int64_t blocks = nx * nq * S;
+    if (nvy % KC != 0) return VP_ERR_UNSUPPORTED;
+    const int nq = static_cast<int>(nvy / KC);
+    const int S_ = y_split(nvx, static_cast<int>(sizeof(S)));
+    if (rho && nq * S_ != L.nparts) return VP_ERR_WORKSPACE;  // the layout was computed for another split
+    const int64_t blocks = nx * nq * S_;
     const int64_t a_stride = ny * nvx * nvy, row_stride = nvx * nvy, loop_stride = nvy;
     const int nvel = static_cast<int>(nvy), lt = static_cast<int>(nvx);
-#define VP_Y(RV, RH)                                                                                      \
-    launch_drift<RV, 1, RH, true>(fin, fout, blocks, nq, S, a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, \
-                                  nvel, L.partial, s)
-    if (ny == 128) {
-        if (rho) VP_Y(4, true); else VP_Y(4, false);
-    } else if (ny == 64) {
-        if (rho) VP_Y(2, true); else VP_Y(2, false);
-    } else if (ny == 32) {
-        if (rho) VP_Y(1, true); else VP_Y(1, false);
+    int rc = VP_ERR_UNSUPPORTED;
+#define VP_Y(N, RH)                                                                                              \
+    if (ny == N && (N) % KG == 0) {                                                                              \
+        constexpr int RV = (N) / KG > 0 ? (N) / KG : 1;                                                          \
+        rc = launch_drift<S, RV, 1, RH, true>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride, lt,   \
+                                              L.Wy, L.ILy, nvel, L.partial, s);                                  \
+    }
+    if (rho) {
+        VP_Y(128, true) else VP_Y(64, true) else VP_Y(32, true)
     } else {
-        return VP_ERR_UNSUPPORTED;
+        VP_Y(128, false) else VP_Y(64, false) else VP_Y(32, false)
     }
 #undef VP_Y
+    if (rc != VP_OK) return rc;
     VP_CHECK_LAUNCH();
     return VP_OK;
+}
+
+int fast2d_xdrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, int passes,
+                  hipStream_t s) {
+    return xdrift<double>(fin, fout, shape, L, passes, s);
+}
+int fast2d_xdrift(const float* fin, float* fout, const int64_t shape[4], const Fast2DWs& L, int passes,
+                  hipStream_t s) {
+    return xdrift<float>(fin, fout, shape, L, passes, s);
+}
+int fast2d_ydrift(const double* fin, double* fout, const int64_t shape[4], const Fast2DWs& L, bool rho,
+                  hipStream_t s) {
+    return ydrift<double>(fin, fout, shape, L, rho, s);
+}
+int fast2d_ydrift(const float* fin, float* fout, const int64_t shape[4], const Fast2DWs& L, bool rho,
+                  hipStream_t s) {
+    return ydrift<float>(fin, fout, shape, L, rho, s);
 }
 
 template <int NVX, int NVY, bool PF>
@@ -750,17 +873,17 @@ static int launch_kick(const double* fin, double* fout, int nplanes, const doubl
     return VP_OK;
 }
 
-template <int NVX, int NVY>
-static int launch_kick_stream(const double* fin, double* fout, int nplanes, const double* Ex, const double* Ey,
-                              double dt, double dvx, double dvy, hipStream_t s) {
+template <typename S, int NVX, int NVY>
+static int launch_kick_stream(const S* fin, S* fout, int nplanes, const double* Ex, const double* Ey, double dt,
+                              double dvx, double dvy, hipStream_t s) {
     constexpr int PPW = 128 / NVY;
     if (nplanes % PPW != 0) return VP_ERR_UNSUPPORTED;
     const int grid = nplanes / PPW;
     const bool pf16 = knob(VP_KNOB_KICK_PREFETCH) == 2;
     const bool nt = (nt_mask() & 4) != 0;
-#define VP_K(PF, NT)                                                                                           \
-    hipLaunchKernelGGL((kick_stream_kernel<NVX, NVY, PF, NT>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, Ex, \
-                       Ey, dt, dvx, dvy)
+#define VP_K(PF, NT)                                                                                              \
+    hipLaunchKernelGGL((kick_stream_kernel<S, NVX, NVY, PF, NT>), dim3(grid), dim3(128), 0, s, fin, fout, nplanes, \
+                       Ex, Ey, dt, dvx, dvy)
     if (pf16) {
         if (nt) VP_K(16, true); else VP_K(16, false);
     } else {
@@ -771,6 +894,26 @@ static int launch_kick_stream(const double* fin, double* fout, int nplanes, cons
     return VP_OK;
 }
 
+template <typename S>
+static int kick_stream(const S* fin, S* fout, const int64_t shape[4], const double* Ex, const double* Ey, double dt,
+                       double dvx, double dvy, hipStream_t s) {
+    const int64_t nvx = shape[2], nvy = shape[3];
+    const int nplanes = static_cast<int>(shape[0] * shape[1]);
+#define VP_KS(A, B) \
+    if (nvx == A && nvy == B) return launch_kick_stream<S, A, B>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+    VP_KS(128, 128)
+    VP_KS(128, 64)
+    VP_KS(128, 32)
+    VP_KS(64, 128)
+    VP_KS(64, 64)
+    VP_KS(64, 32)
+    VP_KS(32, 128)
+    VP_KS(32, 64)
+    VP_KS(32, 32)
+#undef VP_KS
+    return VP_ERR_UNSUPPORTED;
+}
+
 int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const double* Ex, const double* Ey,
                 double dt, double dvx, double dvy, hipStream_t s) {
     const int64_t nvx = shape[2], nvy = shape[3];
@@ -778,21 +921,8 @@ int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const d
     const int64_t cells = shape[0] * shape[1] * nvx * nvy;
     const bool disjoint = fout + cells <= fin || fin + cells <= fout;
     if (disjoint && knob(VP_KNOB_KICK_KERNEL) != 1) {
-#define VP_KS(A, B)                                                                                  \
-    if (nvx == A && nvy == B) {                                                                      \
-        const int rc = launch_kick_stream<A, B>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);        \
-        if (rc != VP_ERR_UNSUPPORTED) return rc;                                                     \
-    }
-        VP_KS(128, 128)
-        VP_KS(128, 64)
-        VP_KS(128, 32)
-        VP_KS(64, 128)
-        VP_KS(64, 64)
-        VP_KS(64, 32)
-        VP_KS(32, 128)
-        VP_KS(32, 64)
-        VP_KS(32, 32)
-#undef VP_KS
+        const int rc = kick_stream<double>(fin, fout, shape, Ex, Ey, dt, dvx, dvy, s);
+        if (rc != VP_ERR_UNSUPPORTED) return rc;
     }
     const bool pf = knob(VP_KNOB_KICK_PREFETCH) != 0;
 #define VP_K(A, B)                                                                                  \
@@ -810,6 +940,15 @@ int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const d
     VP_K(32, 32)
 #undef VP_K
     return VP_ERR_UNSUPPORTED;
+}
+
+// FP32 storage: row-streaming kick only (f_in and f_out must not overlap).
+int fast2d_kick(const float* fin, float* fout, const int64_t shape[4], const double* Ex, const double* Ey, double dt,
+                double dvx, double dvy, hipStream_t s) {
+    const int64_t cells = shape[0] * shape[1] * shape[2] * shape[3];
+    const bool disjoint = fout + cells <= fin || fin + cells <= fout;
+    if (!disjoint) return VP_ERR_UNSUPPORTED;
+    return kick_stream<float>(fin, fout, shape, Ex, Ey, dt, dvx, dvy, s);
 }
 
 }  // namespace vp

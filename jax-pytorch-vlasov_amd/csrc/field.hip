@@ -51,10 +51,13 @@ __global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restric
     const int nk = nx / 2 + 1;
     for (int k = threadIdx.x; k < nk; k += blockDim.x) {
         double re = 0.0, im = 0.0;
+        int ti = 0;  // (k * x) mod nx, stepped
         for (int x = 0; x < nx; ++x) {
-            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            const double2 t = tw[ti];
             re += rho[x] * t.x;
             im -= rho[x] * t.y;
+            ti += k;
+            ti -= (ti >= nx) ? nx : 0;
         }
         // E_k = -i rho_k inv_kx ; phi_k = rho_k inv_kx^2   (vlasov1D1V.py:556-567)
         Ek[k] = make_double2(im * inv_kx[k], -re * inv_kx[k]);
@@ -65,10 +68,13 @@ __global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restric
     const double inv_n = 1.0 / static_cast<double>(nx);
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
         double se = Ek[0].x, sp = Pk[0].x;
+        int ti = x % nx;  // (x * k) mod nx, stepped
         for (int k = 1; k < khalf; ++k) {
-            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            const double2 t = tw[ti];
             se += 2.0 * (Ek[k].x * t.x - Ek[k].y * t.y);
             sp += 2.0 * (Pk[k].x * t.x - Pk[k].y * t.y);
+            ti += x;
+            ti -= (ti >= nx) ? nx : 0;
         }
         if ((nx & 1) == 0) {
             const double sgn = (x & 1) ? -1.0 : 1.0;
@@ -85,7 +91,8 @@ __global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restric
 // ws: R[nx][nky], T[3][nx][nky]  (double2), nky = ny/2 + 1
 // ---------------------------------------------------------------------------
 // rho comes either from `rho` or, when `partial` is given, from nparts
-// compensated partial sums per point (merged in order) times dvx, then dvy.
+// compensated partial sums per point (layout [part][x * ny + y], merged in
+// part order) times dvx, then dvy.
 __global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ rho, const double2* __restrict__ partial,
                                                     int nparts, double dvx, double dvy, int nx, int ny,
                                                     double* __restrict__ delta_rho, double2* __restrict__ R) {
@@ -97,9 +104,10 @@ __global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ r
     for (int y = threadIdx.x; y < ny; y += blockDim.x) {
         double rv;
         if (partial) {
-            const double2* pp = partial + (static_cast<int64_t>(x) * ny + y) * nparts;
+            const int64_t npoints = static_cast<int64_t>(nx) * ny;
+            const double2* pp = partial + static_cast<int64_t>(x) * ny + y;  // layout [part][x * ny + y]
             KSum k{0.0, 0.0};
-            for (int i = 0; i < nparts; ++i) ksum_merge(k, KSum{pp[i].x, pp[i].y});
+            for (int i = 0; i < nparts; ++i) ksum_merge(k, KSum{pp[i * npoints].x, pp[i * npoints].y});
             rv = ((k.s + k.c) * dvx) * dvy;
         } else {
             rv = rho[static_cast<int64_t>(x) * ny + y];
@@ -112,10 +120,13 @@ __global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ r
     const int nky = ny / 2 + 1;
     for (int k = threadIdx.x; k < nky; k += blockDim.x) {
         double re = 0.0, im = 0.0;
+        int ti = 0;  // (k * y) mod ny, stepped
         for (int y = 0; y < ny; ++y) {
-            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * y) % ny)];
+            const double2 t = tw[ti];
             re += row[y] * t.x;
             im -= row[y] * t.y;
+            ti += k;
+            ti -= (ti >= ny) ? ny : 0;
         }
         R[static_cast<int64_t>(x) * nky + k] = make_double2(re, im);
     }
@@ -136,11 +147,14 @@ __global__ __launch_bounds__(256) void p2d_cols(const double2* __restrict__ R, i
     const double kyv = ky[ky_i];
     for (int k = threadIdx.x; k < nx; k += blockDim.x) {
         double re = 0.0, im = 0.0;
+        int ti = 0;  // (k * x) mod nx, stepped
         for (int x = 0; x < nx; ++x) {
-            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            const double2 t = tw[ti];
             // (a + ib)(c - is) = (ac + bs) + i(bc - as)
             re += col[x].x * t.x + col[x].y * t.y;
             im += col[x].y * t.x - col[x].x * t.y;
+            ti += k;
+            ti -= (ti >= nx) ? nx : 0;
         }
         const double iks = inv_ksq[static_cast<int64_t>(k) * nky + ky_i];
         const double pr = re * iks, pi = im * iks;
@@ -156,11 +170,14 @@ __global__ __launch_bounds__(256) void p2d_cols(const double2* __restrict__ R, i
         const int x = idx - q * nx;
         const double2* Qq = Q + q * nx;
         double re = 0.0, im = 0.0;
+        int ti = 0;  // (x * k) mod nx, stepped
         for (int k = 0; k < nx; ++k) {
-            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * x) % nx)];
+            const double2 t = tw[ti];
             // (a + ib)(c + is) = (ac - bs) + i(bc + as)
             re += Qq[k].x * t.x - Qq[k].y * t.y;
             im += Qq[k].y * t.x + Qq[k].x * t.y;
+            ti += x;
+            ti -= (ti >= nx) ? nx : 0;
         }
         T[(static_cast<int64_t>(q) * nx + x) * nky + ky_i] = make_double2(re * inv_n, im * inv_n);
     }
@@ -185,9 +202,12 @@ __global__ __launch_bounds__(256) void p2d_rows_inv(const double2* __restrict__ 
     const double inv_n = 1.0 / static_cast<double>(ny);
     for (int y = threadIdx.x; y < ny; y += blockDim.x) {
         double s = row[0].x;
+        int ti = y % ny;  // (y * k) mod ny, stepped
         for (int k = 1; k < khalf; ++k) {
-            const double2 t = tw[static_cast<int>((static_cast<int64_t>(k) * y) % ny)];
+            const double2 t = tw[ti];
             s += 2.0 * (row[k].x * t.x - row[k].y * t.y);
+            ti += y;
+            ti -= (ti >= ny) ? ny : 0;
         }
         if ((ny & 1) == 0) s += row[ny / 2].x * ((y & 1) ? -1.0 : 1.0);
         out[static_cast<int64_t>(x) * ny + y] = s * inv_n;

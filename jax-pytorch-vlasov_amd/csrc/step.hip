@@ -97,18 +97,32 @@ static size_t generic_2d_ws(const int64_t shape[4]) {
            vp::poisson2d_ws(shape[0], shape[1]);
 }
 
-static size_t fused_2d_ws(const int64_t shape[4]) {
-    return vp::fast2d_layout(shape, nullptr).bytes + vp::poisson2d_ws(shape[0], shape[1]);
+// with_ftmp: the run / full-step layout (adds the kick's f-sized buffer);
+// without: the kernel-level pieces (the caller owns the second f buffer).
+static size_t fused_2d_ws(const int64_t shape[4], int elem_bytes, bool with_ftmp) {
+    return vp::fast2d_layout(shape, nullptr, elem_bytes, with_ftmp).bytes;
 }
 
 extern "C" size_t vp_step_2d2v_workspace(const int64_t shape[4]) {
     if (!shape) return 0;
-    return vp::fast2d_supported(shape) ? fused_2d_ws(shape) : generic_2d_ws(shape);
+    return vp::fast2d_supported(shape) ? fused_2d_ws(shape, 8, true) : generic_2d_ws(shape);
 }
 
 extern "C" size_t vp_run_2d2v_workspace(const int64_t shape[4]) {
     if (!shape || !vp::fast2d_supported(shape)) return 0;
-    return fused_2d_ws(shape);
+    return fused_2d_ws(shape, 8, true);
+}
+
+extern "C" size_t vp_run_2d2v_f32_workspace(const int64_t shape[4]) {
+    if (!shape || !vp::fast2d_supported_f32(shape)) return 0;
+    return fused_2d_ws(shape, 4, true);
+}
+
+extern "C" size_t vp_fused_workspace(const int64_t shape[4], int elem_bytes) {
+    if (!shape) return 0;
+    if (elem_bytes == 8) return vp::fast2d_supported(shape) ? fused_2d_ws(shape, 8, false) : 0;
+    if (elem_bytes == 4) return vp::fast2d_supported_f32(shape) ? fused_2d_ws(shape, 4, false) : 0;
+    return 0;
 }
 
 extern "C" int vp_field_energy_f64(const double* Ex, const double* Ey, int64_t n, double dx, double dy,
@@ -122,8 +136,8 @@ static int fused_step_2d(const double* f_in, double* f_out, const int64_t shape[
                          double dy, double dvx, double dvy, int physics_mode, double* Ex, double* Ey,
                          double* delta_rho, double* phi, void* ws, hipStream_t s) {
     const int64_t nx = shape[0], ny = shape[1];
-    vp::Fast2DWs L = vp::fast2d_layout(shape, ws);
-    void* pws = static_cast<unsigned char*>(ws) + L.bytes;
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws, 8, true);
+    void* pws = L.pws;
     const double half = dt / 2.0;
     VP_TRY(vp::fast2d_tables(shape, L, vx, vy, half, dx, dy, s));
     VP_TRY(vp::fast2d_xdrift(f_in, f_out, shape, L, 1, s));
@@ -137,24 +151,29 @@ static int fused_step_2d(const double* f_in, double* f_out, const int64_t shape[
         VP_TRY(vp::fill(delta_rho, nx * ny, 0.0, s));
         VP_TRY(vp::fill(phi, nx * ny, 0.0, s));
     }
-    VP_TRY(vp::fast2d_kick(f_out, L.ftmp, shape, Ex, Ey, dt, dvx, dvy, s));
-    VP_TRY(vp::fast2d_ydrift(L.ftmp, f_out, shape, L, false, s));
+    double* tmp = static_cast<double*>(L.ftmp);
+    VP_TRY(vp::fast2d_kick(f_out, tmp, shape, Ex, Ey, dt, dvx, dvy, s));
+    VP_TRY(vp::fast2d_ydrift(tmp, f_out, shape, L, false, s));
     VP_TRY(vp::fast2d_xdrift(f_out, f_out, shape, L, 1, s));
     return VP_OK;
 }
 
 // Kernel-level pieces of the fused step, in the order vp_run_2d2v_f64 issues
-// them; ws is laid out as for vp_run_2d2v_workspace.
+// them; ws is laid out as for vp_fused_workspace (a prefix of the run layout).
 #define VP_FUSED_GUARD(ws, ws_bytes)                                   \
     VP_RETURN_IF(!shape || !ws, VP_ERR_ARG);                           \
     VP_RETURN_IF(!vp::fast2d_supported(shape), VP_ERR_UNSUPPORTED);    \
-    VP_RETURN_IF(ws_bytes < fused_2d_ws(shape), VP_ERR_WORKSPACE)
+    VP_RETURN_IF(ws_bytes < fused_2d_ws(shape, 8, false), VP_ERR_WORKSPACE)
+#define VP_FUSED_GUARD_F32(ws, ws_bytes)                                \
+    VP_RETURN_IF(!shape || !ws, VP_ERR_ARG);                            \
+    VP_RETURN_IF(!vp::fast2d_supported_f32(shape), VP_ERR_UNSUPPORTED); \
+    VP_RETURN_IF(ws_bytes < fused_2d_ws(shape, 4, false), VP_ERR_WORKSPACE)
 
 extern "C" int vp_fused_tables_f64(const int64_t shape[4], const double* vx, const double* vy, double half_dt,
                                    double dx, double dy, void* ws, size_t ws_bytes, vp_stream_t stream) {
     VP_FUSED_GUARD(ws, ws_bytes);
     VP_RETURN_IF(!vx || !vy, VP_ERR_ARG);
-    return vp::fast2d_tables(shape, vp::fast2d_layout(shape, ws), vx, vy, half_dt, dx, dy,
+    return vp::fast2d_tables(shape, vp::fast2d_layout(shape, ws, 8, false), vx, vy, half_dt, dx, dy,
                              static_cast<hipStream_t>(stream));
 }
 
@@ -162,7 +181,7 @@ extern "C" int vp_fused_xdrift_f64(const double* f_in, double* f_out, const int6
                                    size_t ws_bytes, vp_stream_t stream) {
     VP_FUSED_GUARD(ws, ws_bytes);
     VP_RETURN_IF(!f_in || !f_out || (passes != 1 && passes != 2), VP_ERR_ARG);
-    return vp::fast2d_xdrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws), passes,
+    return vp::fast2d_xdrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws, 8, false), passes,
                              static_cast<hipStream_t>(stream));
 }
 
@@ -170,7 +189,7 @@ extern "C" int vp_fused_ydrift_f64(const double* f_in, double* f_out, const int6
                                    void* ws, size_t ws_bytes, vp_stream_t stream) {
     VP_FUSED_GUARD(ws, ws_bytes);
     VP_RETURN_IF(!f_in || !f_out, VP_ERR_ARG);
-    return vp::fast2d_ydrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws), with_rho != 0,
+    return vp::fast2d_ydrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws, 8, false), with_rho != 0,
                              static_cast<hipStream_t>(stream));
 }
 
@@ -178,11 +197,9 @@ extern "C" int vp_fused_poisson_f64(const int64_t shape[4], double dvx, double d
                                     const double* ky, const double* inv_ksq, double* Ex, double* Ey,
                                     double* delta_rho, double* phi, void* ws, size_t ws_bytes, vp_stream_t stream) {
     VP_FUSED_GUARD(ws, ws_bytes);
-    vp::Fast2DWs L = vp::fast2d_layout(shape, ws);
-    void* pws = static_cast<unsigned char*>(ws) + L.bytes;
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws, 8, false);
     return vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, shape[0], shape[1], kx, ky, inv_ksq, Ex, Ey,
-                              delta_rho, phi, pws, vp::poisson2d_ws(shape[0], shape[1]),
-                              static_cast<hipStream_t>(stream));
+                              delta_rho, phi, L.pws, L.pws_bytes, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_t shape[4], const double* Ex,
@@ -196,21 +213,15 @@ extern "C" int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_
 // nsteps Strang steps in place on f.  Between two steps the closing X(dt/2)
 // and the opening X(dt/2) run in one pass (applied twice in LDS): the
 // result is identical to nsteps separate steps.
-extern "C" int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, double dt, const double* vx,
-                               const double* vy, const double* kx, const double* ky, const double* inv_ksq,
-                               double dx, double dy, double dvx, double dvy, int physics_mode, double* Ex,
-                               double* Ey, double* delta_rho, double* phi, double* energy, void* ws,
-                               size_t ws_bytes, vp_stream_t stream) {
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    VP_RETURN_IF(!f || !shape || !vx || !vy || !Ex || !Ey || !delta_rho || !phi || !ws, VP_ERR_ARG);
-    VP_RETURN_IF(physics_mode && (!kx || !ky || !inv_ksq), VP_ERR_ARG);
-    VP_RETURN_IF(nsteps < 0, VP_ERR_ARG);
-    VP_RETURN_IF(!vp::fast2d_supported(shape), VP_ERR_UNSUPPORTED);
-    VP_RETURN_IF(ws_bytes < fused_2d_ws(shape), VP_ERR_WORKSPACE);
-    if (nsteps == 0) return VP_OK;
+template <typename S>
+static int run_2d2v(S* f, const int64_t shape[4], int nsteps, double dt, const double* vx, const double* vy,
+                    const double* kx, const double* ky, const double* inv_ksq, double dx, double dy, double dvx,
+                    double dvy, int physics_mode, double* Ex, double* Ey, double* delta_rho, double* phi,
+                    double* energy, void* ws, hipStream_t s) {
     const int64_t nx = shape[0], ny = shape[1];
-    vp::Fast2DWs L = vp::fast2d_layout(shape, ws);
-    void* pws = static_cast<unsigned char*>(ws) + L.bytes;
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws, static_cast<int>(sizeof(S)), true);
+    void* pws = L.pws;
+    S* tmp = static_cast<S*>(L.ftmp);
     const double half = dt / 2.0;
     VP_TRY(vp::fast2d_tables(shape, L, vx, vy, half, dx, dy, s));
     if (!physics_mode) {
@@ -226,11 +237,82 @@ extern "C" int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, do
             VP_TRY(vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, nx, ny, kx, ky, inv_ksq, Ex, Ey,
                                       delta_rho, phi, pws, vp::poisson2d_ws(nx, ny), s));
         if (energy) VP_TRY(vp::field_energy(Ex, Ey, nx * ny, dx, dy, energy + st, s));
-        VP_TRY(vp::fast2d_kick(f, L.ftmp, shape, Ex, Ey, dt, dvx, dvy, s));
-        VP_TRY(vp::fast2d_ydrift(L.ftmp, f, shape, L, false, s));
+        VP_TRY(vp::fast2d_kick(f, tmp, shape, Ex, Ey, dt, dvx, dvy, s));
+        VP_TRY(vp::fast2d_ydrift(tmp, f, shape, L, false, s));
         VP_TRY(vp::fast2d_xdrift(f, f, shape, L, st == nsteps - 1 ? 1 : 2, s));
     }
     return VP_OK;
+}
+
+#define VP_RUN_CHECKS(SUPPORTED, WS)                                                                           \
+    VP_RETURN_IF(!f || !shape || !vx || !vy || !Ex || !Ey || !delta_rho || !phi || !ws, VP_ERR_ARG);          \
+    VP_RETURN_IF(physics_mode && (!kx || !ky || !inv_ksq), VP_ERR_ARG);                                       \
+    VP_RETURN_IF(nsteps < 0, VP_ERR_ARG);                                                                    \
+    VP_RETURN_IF(!(SUPPORTED), VP_ERR_UNSUPPORTED);                                                           \
+    VP_RETURN_IF(ws_bytes < (WS), VP_ERR_WORKSPACE);                                                         \
+    if (nsteps == 0) return VP_OK
+
+extern "C" int vp_run_2d2v_f64(double* f, const int64_t shape[4], int nsteps, double dt, const double* vx,
+                               const double* vy, const double* kx, const double* ky, const double* inv_ksq,
+                               double dx, double dy, double dvx, double dvy, int physics_mode, double* Ex,
+                               double* Ey, double* delta_rho, double* phi, double* energy, void* ws,
+                               size_t ws_bytes, vp_stream_t stream) {
+    VP_RUN_CHECKS(vp::fast2d_supported(shape), fused_2d_ws(shape, 8, true));
+    return run_2d2v<double>(f, shape, nsteps, dt, vx, vy, kx, ky, inv_ksq, dx, dy, dvx, dvy, physics_mode, Ex, Ey,
+                            delta_rho, phi, energy, ws, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_run_2d2v_f32(float* f, const int64_t shape[4], int nsteps, double dt, const double* vx,
+                               const double* vy, const double* kx, const double* ky, const double* inv_ksq,
+                               double dx, double dy, double dvx, double dvy, int physics_mode, double* Ex,
+                               double* Ey, double* delta_rho, double* phi, double* energy, void* ws,
+                               size_t ws_bytes, vp_stream_t stream) {
+    VP_RUN_CHECKS(vp::fast2d_supported_f32(shape), fused_2d_ws(shape, 4, true));
+    return run_2d2v<float>(f, shape, nsteps, dt, vx, vy, kx, ky, inv_ksq, dx, dy, dvx, dvy, physics_mode, Ex, Ey,
+                           delta_rho, phi, energy, ws, static_cast<hipStream_t>(stream));
+}
+#undef VP_RUN_CHECKS
+
+// Pieces of vp_run_2d2v_f32; ws laid out as vp_fused_workspace(shape, 4).
+extern "C" int vp_fused_tables_f32(const int64_t shape[4], const double* vx, const double* vy, double half_dt,
+                                   double dx, double dy, void* ws, size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD_F32(ws, ws_bytes);
+    VP_RETURN_IF(!vx || !vy, VP_ERR_ARG);
+    return vp::fast2d_tables(shape, vp::fast2d_layout(shape, ws, 4, false), vx, vy, half_dt, dx, dy,
+                             static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_xdrift_f32(const float* f_in, float* f_out, const int64_t shape[4], int passes, void* ws,
+                                   size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD_F32(ws, ws_bytes);
+    VP_RETURN_IF(!f_in || !f_out || (passes != 1 && passes != 2), VP_ERR_ARG);
+    return vp::fast2d_xdrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws, 4, false), passes,
+                             static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_ydrift_f32(const float* f_in, float* f_out, const int64_t shape[4], int with_rho,
+                                   void* ws, size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD_F32(ws, ws_bytes);
+    VP_RETURN_IF(!f_in || !f_out, VP_ERR_ARG);
+    return vp::fast2d_ydrift(f_in, f_out, shape, vp::fast2d_layout(shape, ws, 4, false), with_rho != 0,
+                             static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_poisson_f32(const int64_t shape[4], double dvx, double dvy, const double* kx,
+                                    const double* ky, const double* inv_ksq, double* Ex, double* Ey,
+                                    double* delta_rho, double* phi, void* ws, size_t ws_bytes, vp_stream_t stream) {
+    VP_FUSED_GUARD_F32(ws, ws_bytes);
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws, 4, false);
+    return vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, shape[0], shape[1], kx, ky, inv_ksq, Ex, Ey,
+                              delta_rho, phi, L.pws, L.pws_bytes, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_fused_kick_f32(const float* f_in, float* f_out, const int64_t shape[4], const double* Ex,
+                                 const double* Ey, double dt, double dvx, double dvy, vp_stream_t stream) {
+    VP_RETURN_IF(!shape || !f_in || !f_out || !Ex || !Ey, VP_ERR_ARG);
+    VP_RETURN_IF(shape[0] < 1 || shape[1] < 1 || shape[0] * shape[1] > 0x7fffffffLL, VP_ERR_SHAPE);
+    VP_RETURN_IF(!vp::fast2d_kick_supported(shape), VP_ERR_UNSUPPORTED);
+    return vp::fast2d_kick(f_in, f_out, shape, Ex, Ey, dt, dvx, dvy, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int vp_step_2d2v_f64(const double* f_in, double* f_out, const int64_t shape[4], double dt,

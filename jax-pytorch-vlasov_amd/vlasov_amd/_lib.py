@@ -45,12 +45,22 @@ _SIGS = {
     "vp_run_2d2v_workspace": (c_sz, [I64x4]),
     "vp_run_2d2v_f64": (c_int, [c_vp, I64x4, c_int, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl,
                                 c_dbl, c_dbl, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "vp_fused_workspace": (c_sz, [I64x4, c_int]),
     "vp_fused_tables_f64": (c_int, [I64x4, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp, c_sz, c_vp]),
     "vp_fused_xdrift_f64": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, c_sz, c_vp]),
     "vp_fused_ydrift_f64": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, c_sz, c_vp]),
     "vp_fused_poisson_f64": (c_int, [I64x4, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                      c_sz, c_vp]),
     "vp_fused_kick_f64": (c_int, [c_vp, c_vp, I64x4, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp]),
+    "vp_run_2d2v_f32_workspace": (c_sz, [I64x4]),
+    "vp_run_2d2v_f32": (c_int, [c_vp, I64x4, c_int, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl,
+                                c_dbl, c_dbl, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "vp_fused_tables_f32": (c_int, [I64x4, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp, c_sz, c_vp]),
+    "vp_fused_xdrift_f32": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, c_sz, c_vp]),
+    "vp_fused_ydrift_f32": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, c_sz, c_vp]),
+    "vp_fused_poisson_f32": (c_int, [I64x4, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                     c_sz, c_vp]),
+    "vp_fused_kick_f32": (c_int, [c_vp, c_vp, I64x4, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp]),
     "vp_field_energy_f64": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
     "vp_tune_set": (c_int, [c_int, c_int]),
     "vp_tune_get": (c_int, [c_int]),

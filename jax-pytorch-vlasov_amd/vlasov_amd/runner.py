@@ -3,14 +3,18 @@
 ``Run2D`` owns the state f, the field arrays and the workspace and advances
 them step after step without allocating.  Two kernel paths:
 
-* fused (nx, ny, nvx, nvy in {32, 64, 128}): one C-ABI call per kernel (so a
-  caller can bracket any of them with events), drifts in place, the kick out
-  of place into a second f-sized buffer, the closing X(dt/2) of a step and
-  the opening X(dt/2) of the next run as one pass;
+* fused (float64: nx, ny, nvx, nvy in {32, 64, 128}; float32 storage: nx,
+  ny, nvy in {64, 128}, nvx in {32, 64, 128}): one C-ABI call per kernel (so
+  a caller can bracket any of them with events), drifts in place, the kick
+  out of place into a second f-sized buffer, the closing X(dt/2) of a step
+  and the opening X(dt/2) of the next run as one pass;
 * generic (any shape): the six separate sweeps of the reference step,
   ping-ponging between two copies of f.
 
-Both paths give the same f as repeated calls of ``vlasov_step_impl``.
+For float64 f both paths give the same f as repeated calls of
+``vlasov_step_impl``.  float32 f is the FP32-storage variant: f stored and
+interpolated in float32, charge density, Poisson solve, fields and field
+energy in float64.
 """
 from __future__ import annotations
 
@@ -24,13 +28,25 @@ GENERIC_KERNELS = ("x_drift", "y_drift", "rho", "poisson", "vx_kick", "vy_kick",
                    "field_energy")
 
 
-def fused_supported(shape) -> bool:
-    """True when the library has fused kernels for this f shape."""
-    return len(shape) == 4 and lib().vp_run_2d2v_workspace(_lib.i64x4(shape)) > 0
+def fused_supported(shape, dtype=torch.float64) -> bool:
+    """True when the library has fused kernels for this f shape and dtype."""
+    if len(shape) != 4:
+        return False
+    q = lib().vp_run_2d2v_f32_workspace if dtype == torch.float32 else lib().vp_run_2d2v_workspace
+    return q(_lib.i64x4(shape)) > 0
+
+
+def _suffix(dtype) -> str:
+    if dtype == torch.float64:
+        return "f64"
+    if dtype == torch.float32:
+        return "f32"
+    raise TypeError(f"f must be float64 or float32 (FP32 storage), got {dtype}")
 
 
 class Run2D:
-    """In-place 2D2V Strang stepping of f[Nx, Ny, Nvx, Nvy] (float64)."""
+    """In-place 2D2V Strang stepping of f[Nx, Ny, Nvx, Nvy] (float64, or
+    float32 for the FP32-storage variant)."""
 
     def __init__(self, grid, f: torch.Tensor, dt: float, physics_mode: bool = True, fused: bool | None = None):
         self.g = grid
@@ -39,9 +55,10 @@ class Run2D:
         dev = f.device
         self.device = dev
         self.shape = tuple(int(n) for n in f.shape)
-        self.fused = fused_supported(self.shape) if fused is None else bool(fused)
-        if self.fused and not fused_supported(self.shape):
-            raise ValueError(f"no fused kernels for shape {self.shape} (every extent must be 32, 64 or 128)")
+        self.sfx = _suffix(f.dtype)
+        self.fused = fused_supported(self.shape, f.dtype) if fused is None else bool(fused)
+        if self.fused and not fused_supported(self.shape, f.dtype):
+            raise ValueError(f"no fused kernels for shape {self.shape} and {f.dtype}")
         Nx, Ny = self.shape[0], self.shape[1]
         self.f = [f.contiguous().clone(), torch.empty_like(f)]
         self.fields = torch.zeros((4, Nx, Ny), dtype=torch.float64, device=dev)  # Ex, Ey, delta_rho, phi
@@ -53,7 +70,9 @@ class Run2D:
         self.iks = grid.inv_ksq.to(f64).contiguous()
         self.c_shape = _lib.i64x4(self.shape)
         if self.fused:
-            self.ws = torch.empty(lib().vp_run_2d2v_workspace(self.c_shape), dtype=torch.uint8, device=dev)
+            # pieces workspace: the kick's second buffer is self.f[1]
+            nbytes = lib().vp_fused_workspace(self.c_shape, 8 if self.sfx == "f64" else 4)
+            self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
             self._tables_ready = False
         else:
             self.rho = torch.empty((Nx, Ny), dtype=f64, device=dev)
@@ -114,40 +133,46 @@ class Run2D:
         g = self.g
         s = stream_handle(self.device)
         L = lib()
+        sfx = self.sfx
+        tables = getattr(L, f"vp_fused_tables_{sfx}")
+        xdrift = getattr(L, f"vp_fused_xdrift_{sfx}")
+        ydrift = getattr(L, f"vp_fused_ydrift_{sfx}")
+        poisson = getattr(L, f"vp_fused_poisson_{sfx}")
+        kick = getattr(L, f"vp_fused_kick_{sfx}")
         fp = self.f[0].data_ptr()
         fp2 = self.f[1].data_ptr()  # the kick writes here; the closing Y drift reads it back
         ws, wsn = self.ws.data_ptr(), self.ws.numel()
         Ex, Ey, drho, phi = (t.data_ptr() for t in self.fields)
         sh = self.c_shape
         if not self._tables_ready:
-            check(L.vp_fused_tables_f64(sh, self.vx.data_ptr(), self.vy.data_ptr(), self.dt / 2.0, g.dx, g.dy, ws,
-                                        wsn, s), "vp_fused_tables_f64")
+            check(tables(sh, self.vx.data_ptr(), self.vy.data_ptr(), self.dt / 2.0, g.dx, g.dy, ws, wsn, s),
+                  "fused tables")
             self._tables_ready = True
         if not self.physics_mode:
             self.fields.zero_()
         t = self._timed
-        t(events, "x_drift", lambda: check(L.vp_fused_xdrift_f64(fp, fp, sh, 1, ws, wsn, s), "xdrift"))
+        t(events, "x_drift", lambda: check(xdrift(fp, fp, sh, 1, ws, wsn, s), "xdrift"))
         for st in range(nsteps):
-            t(events, "y_drift_rho", lambda: check(L.vp_fused_ydrift_f64(fp, fp, sh, int(self.physics_mode), ws,
-                                                                         wsn, s), "ydrift"))
+            t(events, "y_drift_rho", lambda: check(ydrift(fp, fp, sh, int(self.physics_mode), ws, wsn, s),
+                                                   "ydrift"))
             if self.physics_mode:
-                t(events, "poisson", lambda: check(L.vp_fused_poisson_f64(
+                t(events, "poisson", lambda: check(poisson(
                     sh, g.dvx, g.dvy, self.kx.data_ptr(), self.ky.data_ptr(), self.iks.data_ptr(), Ex, Ey, drho,
                     phi, ws, wsn, s), "poisson"))
             if energy is not None:
                 ep = energy.data_ptr() + 8 * st
                 t(events, "field_energy", lambda: check(L.vp_field_energy_f64(
                     Ex, Ey, self.shape[0] * self.shape[1], g.dx, g.dy, ep, s), "field_energy"))
-            t(events, "kick", lambda: check(L.vp_fused_kick_f64(fp, fp2, sh, Ex, Ey, self.dt, g.dvx, g.dvy, s),
-                                            "kick"))
-            t(events, "y_drift", lambda: check(L.vp_fused_ydrift_f64(fp2, fp, sh, 0, ws, wsn, s), "ydrift"))
+            t(events, "kick", lambda: check(kick(fp, fp2, sh, Ex, Ey, self.dt, g.dvx, g.dvy, s), "kick"))
+            t(events, "y_drift", lambda: check(ydrift(fp2, fp, sh, 0, ws, wsn, s), "ydrift"))
             last = st == nsteps - 1
             t(events, "x_drift" if last else "xx_drift",
-              lambda: check(L.vp_fused_xdrift_f64(fp, fp, sh, 1 if last else 2, ws, wsn, s), "xdrift"))
+              lambda: check(xdrift(fp, fp, sh, 1 if last else 2, ws, wsn, s), "xdrift"))
 
     def _adv(self, src, dst, axis, vel, vs, dt, dx, periodic, s):
-        check(lib().vp_advect_f64(src.data_ptr(), dst.data_ptr(), self.c_shape, axis, vel.data_ptr(), vs,
-                                  dt, dx, periodic, s), "vp_advect_f64")
+        fn = getattr(lib(), f"vp_advect_{self.sfx}")
+        check(fn(src.data_ptr(), dst.data_ptr(), self.c_shape, axis, vel.data_ptr(), vs, dt, dx, periodic, s),
+              f"vp_advect_{self.sfx}")
 
     def _step_generic(self, events, energy):
         g = self.g
@@ -160,8 +185,9 @@ class Run2D:
         t(events, "x_drift", lambda: self._adv(a, b, 0, self.vx, self.vs_x, half, g.dx, 1, s))
         t(events, "y_drift", lambda: self._adv(b, a, 1, self.vy, self.vs_y, half, g.dy, 1, s))
         if self.physics_mode:
-            t(events, "rho", lambda: check(lib().vp_row_sum_f64(a.data_ptr(), Nx * Ny, Nvx * Nvy, g.dvx, g.dvy, 0.0,
-                                                                self.rho.data_ptr(), s), "vp_row_sum_f64"))
+            row_sum = getattr(lib(), f"vp_row_sum_{self.sfx}")
+            t(events, "rho", lambda: check(row_sum(a.data_ptr(), Nx * Ny, Nvx * Nvy, g.dvx, g.dvy, 0.0,
+                                                   self.rho.data_ptr(), s), "vp_row_sum"))
             t(events, "poisson", lambda: check(lib().vp_poisson_2d_f64(
                 self.rho.data_ptr(), Nx, Ny, self.kx.data_ptr(), self.ky.data_ptr(), self.iks.data_ptr(),
                 Ex.data_ptr(), Ey.data_ptr(), drho.data_ptr(), phi.data_ptr(), self.pws.data_ptr(),

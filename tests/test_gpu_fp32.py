@@ -1,0 +1,134 @@
+"""GPU tests of the FP32-storage variant (f float32 in HBM, interpolated in
+float32; charge density, Poisson solve, fields and field energy in float64).
+
+* With E = 0 the fused float32 sweeps equal the generic float32 sweeps
+  (same arithmetic: float64 weights rounded once, float32 six-tap sum) bit
+  for bit.
+* With the field on, fused and generic float32 runs differ only through the
+  order of the float64 charge-density sums: <= 1e-6 relative.
+* Against the float64 oracle the float32 trajectory stays within 1e-5
+  relative in f and in the field-energy history (10 steps, 64x64x32x64).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import vlasov_oracle as vo
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    from vlasov_amd import _lib, runner
+    from vlasov_amd import vlasov2D2V as v2
+    return v2, runner, _lib
+
+
+def _setup(shape, seed, L=4 * math.pi, V=6.0):
+    v2, runner, _ = _mods()
+    nx, ny, nvx, nvy = shape
+    G = v2.Grid.create(nx, ny, nvx, nvy, L, L, V, V, device="cuda", dtype=torch.float64)
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx) / nx * 2 * math.pi
+    y = np.arange(ny) / ny * 2 * math.pi
+    vx = np.linspace(-V, V, nvx)
+    vy = np.linspace(-V, V, nvy)
+    pert = 1 + 0.05 * (np.cos(x)[:, None] + np.sin(y)[None, :])
+    fv = np.exp(-0.5 * (vx[:, None] ** 2 + vy[None, :] ** 2)) / (2 * math.pi)
+    f0 = pert[:, :, None, None] * fv[None, None] * (1 + 0.01 * rng.random(shape))
+    return G, f0
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 32, 64), (128, 64, 64, 128), (64, 128, 128, 64)])
+def test_fp32_fused_free_streaming_equals_generic(device, shape):
+    v2, runner, _ = _mods()
+    assert runner.fused_supported(shape, torch.float32)
+    G, f0 = _setup(shape, 1)
+    f = torch.from_numpy(f0).float().to(device)
+    a = runner.Run2D(G, f, 0.1, physics_mode=False, fused=True)
+    b = runner.Run2D(G, f, 0.1, physics_mode=False, fused=False)
+    a.advance(1)
+    a.advance(2)
+    b.advance(3)
+    assert a.state.dtype == torch.float32
+    assert torch.equal(a.state, b.state)
+
+
+def test_fp32_fused_matches_generic_with_field(device):
+    v2, runner, _ = _mods()
+    shape = (64, 64, 64, 64)
+    G, f0 = _setup(shape, 2)
+    f = torch.from_numpy(f0).float().to(device)
+    a = runner.Run2D(G, f, 0.1, physics_mode=True, fused=True)
+    b = runner.Run2D(G, f, 0.1, physics_mode=True, fused=False)
+    ea = torch.zeros(4, dtype=torch.float64, device=device)
+    eb = torch.zeros(4, dtype=torch.float64, device=device)
+    a.advance(4, energy=ea)
+    b.advance(4, energy=eb)
+    fa, fb = a.state.double().cpu().numpy(), b.state.double().cpu().numpy()
+    assert np.max(np.abs(fa - fb)) / np.max(np.abs(fb)) <= 1e-6
+    assert np.max(np.abs(ea.cpu().numpy() - eb.cpu().numpy())) / np.max(eb.cpu().numpy()) <= 1e-6
+
+
+def test_fp32_trajectory_vs_fp64_oracle(device):
+    v2, runner, _ = _mods()
+    shape = (64, 64, 32, 64)
+    G, f0 = _setup(shape, 3)
+    g = vo.grid_2d(*shape, 4 * math.pi, 4 * math.pi, 6.0, 6.0)
+    arrays = tuple(a.cpu().numpy() for a in (G.vx, G.vy, G.kx, G.ky, G.inv_ksq))
+    run = runner.Run2D(G, torch.from_numpy(f0).float().to(device), 0.1, physics_mode=True)
+    assert run.fused
+    energy = torch.zeros(10, dtype=torch.float64, device=device)
+    run.advance(10, energy=energy)
+    f_ref = f0
+    W = []
+    for _ in range(10):
+        f_ref, _, _, Ex, Ey = vo.vlasov_step_2d(f_ref, 0.1, g, True, grid_arrays=arrays)
+        W.append(vo.field_energy_2d(Ex, Ey, g.dx, g.dy))
+    f32 = run.state.double().cpu().numpy()
+    err_f = np.max(np.abs(f32 - f_ref)) / np.max(np.abs(f_ref))
+    err_w = np.max(np.abs(energy.cpu().numpy() - np.array(W))) / max(W)
+    # the float32 input rounding alone is ~6e-8 relative
+    assert err_f <= 1e-5, err_f
+    assert err_w <= 1e-5, err_w
+
+
+def test_fp32_xdrift_two_passes_equal_two_calls(device):
+    v2, runner, L = _mods()
+    shape = (128, 64, 32, 64)
+    G, f0 = _setup(shape, 4)
+    f = torch.from_numpy(f0).float().to(device)
+    sh = L.i64x4(shape)
+    lib = L.lib()
+    ws = torch.empty(lib.vp_fused_workspace(sh, 4), dtype=torch.uint8, device=device)
+    s = torch.cuda.current_stream().cuda_stream
+    L.check(lib.vp_fused_tables_f32(sh, G.vx.data_ptr(), G.vy.data_ptr(), 0.05, G.dx, G.dy, ws.data_ptr(),
+                                    ws.numel(), s), "tables")
+    one = f.clone()
+    for _ in range(2):
+        L.check(lib.vp_fused_xdrift_f32(one.data_ptr(), one.data_ptr(), sh, 1, ws.data_ptr(), ws.numel(), s), "x1")
+    two = torch.empty_like(f)
+    L.check(lib.vp_fused_xdrift_f32(f.data_ptr(), two.data_ptr(), sh, 2, ws.data_ptr(), ws.numel(), s), "x2")
+    assert torch.equal(one, two)
+    gen = f.clone()
+    tmp = torch.empty_like(f)
+    vs = L.i64x4((0, 0, 1, 0))
+    for src, dst in ((gen, tmp), (tmp, gen)):
+        L.check(lib.vp_advect_f32(src.data_ptr(), dst.data_ptr(), sh, 0, G.vx.data_ptr(), vs, 0.05, G.dx, 1, s),
+                "advect_f32")
+    assert torch.equal(one, gen)
+
+
+def test_fp32_rejects_in_place_kick_and_small_shapes(device):
+    v2, runner, L = _mods()
+    lib = L.lib()
+    assert lib.vp_run_2d2v_f32_workspace(L.i64x4((32, 64, 64, 64))) == 0
+    assert lib.vp_run_2d2v_f32_workspace(L.i64x4((64, 64, 64, 32))) == 0
+    shape = (64, 64, 32, 64)
+    f = torch.zeros(shape, dtype=torch.float32, device=device)
+    E = torch.zeros(shape[:2], dtype=torch.float64, device=device)
+    rc = lib.vp_fused_kick_f32(f.data_ptr(), f.data_ptr(), L.i64x4(shape), E.data_ptr(), E.data_ptr(), 0.1, 0.1,
+                               0.1, torch.cuda.current_stream().cuda_stream)
+    assert rc == -4  # VP_ERR_UNSUPPORTED: FP32 kick runs out of place only

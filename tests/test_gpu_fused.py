@@ -94,7 +94,7 @@ def test_fused_xdrift_two_passes_equal_two_calls(device):
     f = torch.rand(shape, dtype=torch.float64, device=device)
     sh = L.i64x4(shape)
     lib = L.lib()
-    ws = torch.empty(lib.vp_run_2d2v_workspace(sh), dtype=torch.uint8, device=device)
+    ws = torch.empty(lib.vp_fused_workspace(sh, 8), dtype=torch.uint8, device=device)
     s = torch.cuda.current_stream().cuda_stream
     L.check(lib.vp_fused_tables_f64(sh, G.vx.data_ptr(), G.vy.data_ptr(), 0.05, G.dx, G.dy, ws.data_ptr(),
                                     ws.numel(), s), "tables")
