@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (ROOT, os.path.join(ROOT, "jax-pytorch-vlasov_amd")):
     if p not in sys.path:
@@ -41,6 +43,8 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage comparison")
     ap.add_argument("--fp32-error-steps", type=int, default=300,
                     help="steps of the FP32-storage vs float64 error comparison (default 300)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 512-instance 1D1V ensemble")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--stream-only", action="store_true",
                     help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
     return ap.parse_args()
@@ -129,6 +133,64 @@ def fp32_storage(grid, f0, steps: int, warmup: int, err_steps: int) -> dict:
         "field_energy_last_fp64": float(w64[-1]),
     }
     return res
+
+
+def ensemble_bench(steps: int, n: int = 512, seed: int = 0, sample: int = 8, sample_steps: int = 10) -> dict:
+    """Batched 1D1V ensemble: n seeded instances (configs.ensemble_specs),
+    dt = 0.1, `steps` steps, launch shapes autotuned, run from one captured
+    HIP graph.  Baseline: the plain per-instance loop (one single-instance
+    library step per instance per step) timed on `sample` instances spread
+    over the size range and extrapolated to the whole ensemble."""
+    import torch
+
+    from vlasov_amd import configs
+    from vlasov_amd import vlasov1D1V as v1
+    from vlasov_amd.ensemble import Ensemble
+
+    specs = configs.ensemble_specs(n, seed)
+    ens = Ensemble(specs, 0.1, physics_mode=True)
+    choice = ens.autotune()
+    ens.prepare_graph(steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    energy = ens.run(steps, graph=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    finite = bool(torch.isfinite(energy).all())
+    # plain per-instance loop on a sample, extrapolated per instance-step
+    order = sorted(range(n), key=lambda i: specs[i]["nx"] * specs[i]["nv"])
+    pick = [order[(k * (n - 1)) // max(sample - 1, 1)] for k in range(sample)]
+    per_step = []
+    for i in pick:
+        g = ens.grids[i]
+        f = configs.ensemble_f0(g, specs[i]["k"], specs[i]["amplitude"], specs[i]["drift"])
+        f, *_ = v1.step_with_field(f, 0.1, g.v, g.inv_kx, g.inv_kx_sq, g.dx, g.dv, True)  # warm
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(sample_steps):
+            f, *_ = v1.step_with_field(f, 0.1, g.v, g.inv_kx, g.inv_kx_sq, g.dx, g.dv, True)
+        torch.cuda.synchronize()
+        per_step.append((time.perf_counter() - t1) / sample_steps)
+    base_t = float(np.mean(per_step)) * n * steps
+    return {
+        "metric": "ensemble solves/s (512 seeded 1D1V instances, 500 steps each, FP64)",
+        "value": n / t,
+        "unit": "solves/s",
+        "seconds": t,
+        "instances": n,
+        "steps": steps,
+        "cells": ens.cells,
+        "cell_updates_per_s": ens.cells * steps / t,
+        "energy_finite": finite,
+        "launch_shapes": {k: choice[k] for k in ("ens_cols", "ens_xthreads", "ens_rows")},
+        "graph": "one HIP graph of the whole run (3 launches per step)",
+        "per_instance_loop": {"value": n / base_t, "unit": "solves/s",
+                              "sample": f"{sample} instances across the size range x {sample_steps} steps, "
+                                        "extrapolated per instance-step",
+                              "ms_per_instance_step": float(np.mean(per_step)) * 1e3},
+        "speedup_vs_per_instance_loop": base_t / t,
+        "seed": seed,
+    }
 
 
 def _steps_within(rel_err, tol: float) -> int:
@@ -284,6 +346,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_fp32 and runner.fused_supported(f0.shape, torch.float32):
         out["fp32_storage"] = fp32_storage(grid, f0, args.steps, args.warmup, args.fp32_error_steps)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(args.ensemble_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

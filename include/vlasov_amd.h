@@ -204,6 +204,37 @@ int vp_field_energy_f64(const double* Ex, const double* Ey, int64_t n, double dx
                         double* out, vp_stream_t stream);
 
 /* ---------------------------------------------------------------------
+ * Ensemble of independent 1D1V instances (no reference counterpart: the
+ * reference runs one simulation per process).  Each instance follows
+ * vlasov_step (P1:574-622) on its own f[nx][nv] (C order) inside one pool;
+ * all instances advance together, three launches per step (X drift + rho
+ * partial sums, field solve + field energy, V kick), the closing X(dt/2) of
+ * a step fused with the opening X(dt/2) of the next.  Sweeps are
+ * bit-identical to the single-instance path.
+ *
+ * inst (device array, ninst entries): per instance the offsets of its f in
+ * `pool` and of its tables in `tables` (v[nv] at v_off; inv_kx[nx/2+1] then
+ * inv_kx_sq[nx/2+1] at k_off; both as built by Grid.create, P1:94-122), its
+ * sizes and dx, dv.  Every nx, nv must be >= 6 and <= max_nx <= 256,
+ * max_nv <= 512.  energy (optional, device, [nsteps][ninst]) receives each
+ * step's field energy 1/2 sum(E^2) dx.  Instances may not overlap in pool.
+ * --------------------------------------------------------------------- */
+typedef struct {
+    int64_t f_off;
+    int64_t v_off;
+    int64_t k_off;
+    int32_t nx;
+    int32_t nv;
+    double dx;
+    double dv;
+} vp_ens_inst;
+
+size_t vp_ens_workspace(int ninst, int max_nx, int max_nv);
+int vp_ens_run_f64(double* pool, const vp_ens_inst* inst, int ninst, int max_nx, int max_nv,
+                   const double* tables, int nsteps, double dt, int physics_mode, double* energy,
+                   void* ws, size_t ws_bytes, vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
  * Launch-shape knobs of the fused kernels (no reference counterpart).
  * 0 = automatic.  Process-wide; used by the autotuner and the benchmarks.
  * --------------------------------------------------------------------- */
@@ -221,7 +252,13 @@ enum {
                                   streaming (non-temporal) hint, a bit mask:
                                   1 X drift, 2 Y drift, 4 kick; 8 = none
                                   (0 = 7)                                      */
-    VP_KNOB_COUNT = 6
+    VP_KNOB_ENS_COLS = 6,      /* ensemble X drift: 16 or 32 v columns per
+                                  workgroup (0 = 32)                           */
+    VP_KNOB_ENS_ROWS = 7,      /* ensemble kick: 2, 4, 8 or 16 x rows per
+                                  workgroup (0 = 4)                            */
+    VP_KNOB_ENS_XTHREADS = 8,  /* ensemble X drift: 256, 512 or 1024 threads
+                                  per workgroup (0 = 512)                      */
+    VP_KNOB_COUNT = 9
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

@@ -10,7 +10,7 @@
 // torch.fft / numpy.fft: rfft, then c2c along x; the inverse is c2c along x
 // (scaled 1/nx), then c2r along y (scaled 1/ny) reading only the real parts
 // of the DC and Nyquist bins.
-#include "vp_common.hpp"
+#include "field_dev.hpp"
 
 namespace vp {
 
@@ -25,15 +25,6 @@ __global__ __launch_bounds__(256) void row_sum_kernel(const T* __restrict__ f, i
     if (threadIdx.x == 0) out[blockIdx.x] = ((total * s1) * s2) - shift;
 }
 
-// Twiddle table tw[m] = (cos(2 pi m / n), sin(2 pi m / n)).
-__device__ __forceinline__ void fill_twiddles(double2* tw, int n) {
-    for (int m = threadIdx.x; m < n; m += blockDim.x) {
-        double s, c;
-        sincospi(2.0 * static_cast<double>(m) / static_cast<double>(n), &s, &c);
-        tw[m] = make_double2(c, s);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // 1D: solve_poisson_real (vlasov1D1V.py:538-572), single workgroup.
 // ws: tw[nx], Ek[nx/2+1], Pk[nx/2+1]  (double2)
@@ -43,47 +34,7 @@ __global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restric
                                                          const double* __restrict__ inv_kx_sq,
                                                          double* __restrict__ E, double* __restrict__ phi,
                                                          double2* __restrict__ ws) {
-    double2* tw = ws;
-    double2* Ek = ws + nx;
-    double2* Pk = Ek + (nx / 2 + 1);
-    fill_twiddles(tw, nx);
-    __syncthreads();
-    const int nk = nx / 2 + 1;
-    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
-        double re = 0.0, im = 0.0;
-        int ti = 0;  // (k * x) mod nx, stepped
-        for (int x = 0; x < nx; ++x) {
-            const double2 t = tw[ti];
-            re += rho[x] * t.x;
-            im -= rho[x] * t.y;
-            ti += k;
-            ti -= (ti >= nx) ? nx : 0;
-        }
-        // E_k = -i rho_k inv_kx ; phi_k = rho_k inv_kx^2   (vlasov1D1V.py:556-567)
-        Ek[k] = make_double2(im * inv_kx[k], -re * inv_kx[k]);
-        Pk[k] = make_double2(re * inv_kx_sq[k], im * inv_kx_sq[k]);
-    }
-    __syncthreads();
-    const int khalf = (nx + 1) / 2;  // bins 1..khalf-1 are doubled
-    const double inv_n = 1.0 / static_cast<double>(nx);
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-        double se = Ek[0].x, sp = Pk[0].x;
-        int ti = x % nx;  // (x * k) mod nx, stepped
-        for (int k = 1; k < khalf; ++k) {
-            const double2 t = tw[ti];
-            se += 2.0 * (Ek[k].x * t.x - Ek[k].y * t.y);
-            sp += 2.0 * (Pk[k].x * t.x - Pk[k].y * t.y);
-            ti += x;
-            ti -= (ti >= nx) ? nx : 0;
-        }
-        if ((nx & 1) == 0) {
-            const double sgn = (x & 1) ? -1.0 : 1.0;
-            se += Ek[nx / 2].x * sgn;
-            sp += Pk[nx / 2].x * sgn;
-        }
-        E[x] = se * inv_n;
-        phi[x] = sp * inv_n;
-    }
+    poisson1d_block(rho, nx, inv_kx, inv_kx_sq, E, phi, ws, ws + nx, ws + nx + (nx / 2 + 1));
 }
 
 // ---------------------------------------------------------------------------

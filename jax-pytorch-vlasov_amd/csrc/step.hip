@@ -22,11 +22,6 @@ int field_energy(const double* a, const double* b, int64_t n, double s1, double 
 
 static inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
-#define VP_TRY(expr)              \
-    do {                          \
-        int rc_ = (expr);         \
-        if (rc_ != VP_OK) return rc_; \
-    } while (0)
 
 }  // namespace vp
 

@@ -23,9 +23,18 @@
         if (cond) return (code); \
     } while (0)
 
+#define VP_TRY(expr)                  \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != VP_OK) return rc_; \
+    } while (0)
+
 namespace vp {
 
 constexpr int kWave = 64;
+
+// Launch-shape knob value (vp_tune_set; 0 = automatic), defined in fast2d.hip.
+int knob(int k);
 
 // ---------------------------------------------------------------------------
 // Lagrange weights on nodes -2..3 (lagrange5_interp_sum, vlasov1D1V.py:163-189;

@@ -62,12 +62,15 @@ _SIGS = {
                                      c_sz, c_vp]),
     "vp_fused_kick_f32": (c_int, [c_vp, c_vp, I64x4, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp]),
     "vp_field_energy_f64": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
+    "vp_ens_workspace": (c_sz, [c_int, c_int, c_int]),
+    "vp_ens_run_f64": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_int, c_vp, c_vp, c_sz, c_vp]),
     "vp_tune_set": (c_int, [c_int, c_int]),
     "vp_tune_get": (c_int, [c_int]),
     "vp_stream_f64": (c_int, [c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_vp]),
 }
 
-KNOBS = {"x_split": 0, "y_split": 1, "kick_prefetch": 2, "kick_kernel": 3, "drift_dma": 4, "nontemporal": 5}
+KNOBS = {"x_split": 0, "y_split": 1, "kick_prefetch": 2, "kick_kernel": 3, "drift_dma": 4, "nontemporal": 5,
+         "ens_cols": 6, "ens_rows": 7, "ens_xthreads": 8}
 
 _lib = None
 _lock = threading.Lock()
