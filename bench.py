@@ -45,6 +45,8 @@ def parse():
                     help="steps of the FP32-storage vs float64 error comparison (default 300)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 512-instance 1D1V ensemble")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip training the Fourier-operator surrogate")
+    ap.add_argument("--surrogate-iters", type=int, default=12000)
     ap.add_argument("--stream-only", action="store_true",
                     help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
     return ap.parse_args()
@@ -190,6 +192,51 @@ def ensemble_bench(steps: int, n: int = 512, seed: int = 0, sample: int = 8, sam
                               "ms_per_instance_step": float(np.mean(per_step)) * 1e3},
         "speedup_vs_per_instance_loop": base_t / t,
         "seed": seed,
+    }
+
+
+def surrogate_bench(iters: int, ntrain: int = 20000, ntest: int = 1000, steps: int = 500) -> dict:
+    """Fourier-operator surrogate of log W_E(t): data from the batched solver
+    (128 x 256 grid, 500 steps), trained on the GPU kernels, evaluated on
+    held-out parameters against the solver."""
+    import torch
+
+    from vlasov_amd import surrogate as sg
+
+    ptr, pte = sg.sample_params(ntrain, 1), sg.sample_params(ntest, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Wtr = sg.solve(ptr, steps)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Wte = sg.solve(pte, steps)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    sur = sg.Surrogate(steps)
+    t0 = time.perf_counter()
+    losses = sur.fit(ptr, Wtr, iters=iters)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    sur.predict(pte[:8])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Wp = sur.predict(pte)
+    torch.cuda.synchronize()
+    t_pred = time.perf_counter() - t0
+    e = sg.rel_l2(Wp, Wte).cpu().numpy()
+    elog = (torch.log(Wp) - torch.log(Wte)).abs().mean(dim=1).cpu().numpy()
+    return {
+        "model": "1D Fourier neural operator over time: 4 layers, width 64, 32 modes, BF16 MFMA / FP32 accumulate",
+        "train_runs": ntrain, "test_runs": ntest, "steps": steps, "grid": [128, 256],
+        "data_seconds": t_data, "train_iters": iters, "train_seconds": t_train,
+        "final_loss": losses[-1][1],
+        "test_rel_l2_median": float(np.median(e)), "test_rel_l2_p90": float(np.percentile(e, 90)),
+        "test_frac_within_2pct": float(np.mean(e <= 0.02)), "test_frac_within_10pct": float(np.mean(e <= 0.10)),
+        "test_mean_abs_log_err_median": float(np.median(elog)),
+        "predict_queries_per_s": ntest / t_pred, "solver_queries_per_s": ntest / t_solve,
+        "speedup_vs_solver": t_solve / t_pred,
+        "fallback_rule": "judged relative L2 error of the history > 2% -> the request is answered by the solver",
     }
 
 
@@ -348,6 +395,8 @@ def main():
         out["fp32_storage"] = fp32_storage(grid, f0, args.steps, args.warmup, args.fp32_error_steps)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_bench(args.surrogate_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

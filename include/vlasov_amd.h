@@ -235,6 +235,34 @@ int vp_ens_run_f64(double* pool, const vp_ens_inst* inst, int ninst, int max_nx,
                    void* ws, size_t ws_bytes, vp_stream_t stream);
 
 /* ---------------------------------------------------------------------
+ * Surrogate network kernels (no reference counterpart): the Fourier-operator
+ * surrogate of the field-energy history is trained and evaluated with these.
+ * All tensors are float32 device arrays.
+ *
+ * vp_nn_gemm: for b < batch, C_b(i,j) = alpha sum_k A_b(i,k) B_b(k,j)
+ *   + beta C_b(i,j), i < M, j < N, k < K, with X_b(r,s) = X[b*sXb + r*sXr +
+ *   s*sXs] (a transpose is a stride swap).  A and B are rounded to bfloat16
+ *   (nearest even) and multiplied on the matrix cores with float32
+ *   accumulation.  beta == 0 does not read C.
+ * --------------------------------------------------------------------- */
+int vp_nn_gemm(int batch, int M, int N, int K, float alpha, const float* A, int64_t sAb, int64_t sAi,
+               int64_t sAk, const float* B, int64_t sBb, int64_t sBk, int64_t sBj, float beta, float* C,
+               int64_t sCb, int64_t sCi, int64_t sCj, vp_stream_t stream);
+/* y = x Phi(x) (exact GELU) and its backward dx = dy (Phi(x) + x phi(x)) */
+int vp_nn_gelu(const float* x, float* y, int64_t n, vp_stream_t stream);
+int vp_nn_gelu_bwd(const float* x, const float* dy, float* dx, int64_t n, vp_stream_t stream);
+/* y[r][c] += bias[c];  db[c] = sum_r dy[r][c] */
+int vp_nn_bias_add(float* y, const float* bias, int64_t rows, int cols, vp_stream_t stream);
+int vp_nn_bias_grad(const float* dy, float* db, int64_t rows, int cols, vp_stream_t stream);
+/* grad = 2 (pred - target) / n (grad may be NULL); partial[ceil(n/256)]
+ * receives per-block sums of squared errors */
+int vp_nn_mse(const float* pred, const float* target, float* grad, float* partial, int64_t n,
+              vp_stream_t stream);
+/* Adam with bias correction at step >= 1 (float32 master weights) */
+int vp_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+               float eps, int step, vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
  * Launch-shape knobs of the fused kernels (no reference counterpart).
  * 0 = automatic.  Process-wide; used by the autotuner and the benchmarks.
  * --------------------------------------------------------------------- */

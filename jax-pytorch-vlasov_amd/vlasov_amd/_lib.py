@@ -64,6 +64,15 @@ _SIGS = {
     "vp_field_energy_f64": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
     "vp_ens_workspace": (c_sz, [c_int, c_int, c_int]),
     "vp_ens_run_f64": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_int, c_vp, c_vp, c_sz, c_vp]),
+    "vp_nn_gemm": (c_int, [c_int, c_int, c_int, c_int, ctypes.c_float, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64,
+                           c_i64, ctypes.c_float, c_vp, c_i64, c_i64, c_i64, c_vp]),
+    "vp_nn_gelu": (c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "vp_nn_gelu_bwd": (c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "vp_nn_bias_add": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),
+    "vp_nn_bias_grad": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),
+    "vp_nn_mse": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "vp_nn_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_float, c_int, c_vp]),
     "vp_tune_set": (c_int, [c_int, c_int]),
     "vp_tune_get": (c_int, [c_int]),
     "vp_stream_f64": (c_int, [c_int, c_vp, c_vp, c_i64, c_int, c_int, c_int, c_vp]),

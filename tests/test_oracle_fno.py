@@ -1,0 +1,84 @@
+"""CPU checks that pin the surrogate's float64 oracle (oracle/fno_oracle.py).
+
+* The truncated inverse DFT G undoes the forward DFT F on signals made of
+  the kept modes (closed form), and G F is a projection (idempotent).
+* Every gradient of ``backward`` matches central finite differences of the
+  forward loss (relative 1e-6), for every parameter tensor.
+* With zero spectral weights, the layer reduces to a pointwise MLP that a
+  plain matrix product restates.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import fno_oracle as fo
+
+
+def test_dft_mats_invert_band_limited_signals():
+    T, M = 40, 7
+    Fc, Fs, Gc, Gs = fo.dft_mats(T, M)
+    rng = np.random.default_rng(0)
+    t = np.arange(T)
+    a = rng.standard_normal(M)
+    b = rng.standard_normal(M)
+    b[0] = 0.0
+    x = sum(a[m] * np.cos(2 * math.pi * m * t / T) + b[m] * np.sin(2 * math.pi * m * t / T) for m in range(M))
+    rec = Gc @ (Fc @ x) + Gs @ (Fs @ x)
+    np.testing.assert_allclose(rec, x, atol=1e-12)
+    P = Gc @ Fc + Gs @ Fs
+    np.testing.assert_allclose(P @ P, P, atol=1e-12)
+    # a mode above the cut is removed
+    hi = np.cos(2 * math.pi * (M + 2) * t / T)
+    np.testing.assert_allclose(Gc @ (Fc @ hi) + Gs @ (Fs @ hi), 0.0, atol=1e-12)
+
+
+def test_backward_matches_finite_differences():
+    B, T, cin, C, M, Hd, L = 2, 12, 3, 4, 3, 5, 2
+    p = fo.init_params(cin, C, M, Hd, L, seed=1)
+    for k in p:  # make the spectral weights comparable to the others
+        if k.startswith("Wr") or k.startswith("Wi"):
+            p[k] = p[k] * C * C * 0.3
+    mats = fo.dft_mats(T, M)
+    rng = np.random.default_rng(2)
+    X0 = rng.standard_normal((B, T, cin))
+    target = rng.standard_normal((B, T))
+
+    def loss(pp):
+        y, _ = fo.forward(pp, X0, mats)
+        return fo.mse(y, target)[0]
+
+    y, cache = fo.forward(p, X0, mats)
+    _, dy = fo.mse(y, target)
+    g = fo.backward(p, cache, dy, mats)
+    assert set(g) == set(p)
+    h = 1e-6
+    for name, w in p.items():
+        flat = w.reshape(-1)
+        idx = rng.choice(flat.size, size=min(6, flat.size), replace=False)
+        for i in idx:
+            old = flat[i]
+            flat[i] = old + h
+            lp = loss(p)
+            flat[i] = old - h
+            lm = loss(p)
+            flat[i] = old
+            fd = (lp - lm) / (2 * h)
+            an = g[name].reshape(-1)[i]
+            assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd)), (name, i, fd, an)
+
+
+def test_zero_spectral_weights_is_pointwise_mlp():
+    B, T, cin, C, M, Hd, L = 3, 10, 2, 5, 4, 6, 3
+    p = fo.init_params(cin, C, M, Hd, L, seed=3)
+    for l in range(L):
+        p[f"Wr{l}"][:] = 0.0
+        p[f"Wi{l}"][:] = 0.0
+    X0 = np.random.default_rng(4).standard_normal((B, T, cin))
+    y, _ = fo.forward(p, X0, fo.dft_mats(T, M))
+    H = X0 @ p["P"] + p["bP"]
+    for l in range(L):
+        Z = H @ p[f"W{l}"] + p[f"b{l}"]
+        H = fo.gelu(Z) if l < L - 1 else Z
+    ref = (fo.gelu(H @ p["Q1"] + p["bQ1"]) @ p["Q2"] + p["bQ2"])[..., 0]
+    np.testing.assert_allclose(y, ref, atol=1e-12)
