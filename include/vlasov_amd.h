@@ -203,6 +203,21 @@ int vp_fused_kick_f32(const float* f_in, float* f_out, const int64_t shape[4], c
 int vp_field_energy_f64(const double* Ex, const double* Ey, int64_t n, double dx, double dy,
                         double* out, vp_stream_t stream);
 
+/* Moments of f viewed as [rows][len] (compensated float64 sums):
+ *   out[0] = ((sum f) * s1) * s2                       mass
+ *   out[1] = ((1/2 sum f v^2) * s1) * s2               kinetic energy
+ *   out[2] = ((sum f^2) * s1) * s2                     L2 norm squared
+ * 1D1V: rows = Nx, len = Nv, vx = v, vy = NULL, s1 = dx, s2 = dv.
+ * 2D2V: rows = Nx*Ny, len = Nvx*Nvy, v^2 = vx[j / nvy]^2 + vy[j % nvy]^2,
+ *       s1 = dx*dy*dvx*dvy, s2 = 1.  ws: vp_moments_workspace() bytes. */
+size_t vp_moments_workspace(void);
+int vp_moments_f64(const double* f, int64_t rows, int64_t len, const double* vx, const double* vy,
+                   int64_t nvy, double s1, double s2, double* out, void* ws, size_t ws_bytes,
+                   vp_stream_t stream);
+int vp_moments_f32(const float* f, int64_t rows, int64_t len, const double* vx, const double* vy,
+                   int64_t nvy, double s1, double s2, double* out, void* ws, size_t ws_bytes,
+                   vp_stream_t stream);
+
 /* ---------------------------------------------------------------------
  * Ensemble of independent 1D1V instances (no reference counterpart: the
  * reference runs one simulation per process).  Each instance follows

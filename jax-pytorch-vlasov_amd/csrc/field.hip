@@ -262,6 +262,79 @@ int field_energy(const double* a, const double* b, int64_t n, double s1, double 
     return VP_OK;
 }
 
+// Moments of f [rows][len]: mass sum f, kinetic sum f v2 and L2 sum f^2 with
+// v2 of column j = vx[j / nvy]^2 + vy[j % nvy]^2 (2D2V; vy null: vx[j]^2).
+// Grid-stride compensated sums; partial[block][3] as (sum, compensation).
+template <typename T>
+__global__ __launch_bounds__(256) void moments_partial_kernel(const T* __restrict__ f, int64_t rows, int64_t len,
+                                                              const double* __restrict__ vx,
+                                                              const double* __restrict__ vy, int64_t nvy,
+                                                              double2* __restrict__ partial) {
+    __shared__ double scratch[2 * 256 / kWave];
+    KSum m{0.0, 0.0}, k{0.0, 0.0}, l{0.0, 0.0};
+    const int64_t n = rows * len;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+        const double v = static_cast<double>(f[i]);
+        const int64_t j = i % len;
+        double v2;
+        if (vy) {
+            const double a = vx[j / nvy], b = vy[j % nvy];
+            v2 = a * a + b * b;
+        } else {
+            const double a = vx[j];
+            v2 = a * a;
+        }
+        ksum_add(m, v);
+        ksum_add(k, v * v2);
+        ksum_add(l, v * v);
+    }
+    const double tm = block_ksum(m, scratch);
+    const double tk = block_ksum(k, scratch);
+    const double tl = block_ksum(l, scratch);
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x + 0] = make_double2(tm, 0.0);
+        partial[3 * blockIdx.x + 1] = make_double2(tk, 0.0);
+        partial[3 * blockIdx.x + 2] = make_double2(tl, 0.0);
+    }
+}
+
+// out = (((sum) * s1) * s2, ((0.5 * sum f v2) * s1) * s2, ((sum f^2) * s1) * s2)
+__global__ __launch_bounds__(256) void moments_final_kernel(const double2* __restrict__ partial, int nblocks,
+                                                            double s1, double s2, double* __restrict__ out) {
+    __shared__ double scratch[2 * 256 / kWave];
+    double r[3];
+    for (int q = 0; q < 3; ++q) {
+        KSum k{0.0, 0.0};
+        for (int b = threadIdx.x; b < nblocks; b += 256) ksum_add(k, partial[3 * b + q].x);
+        r[q] = block_ksum(k, scratch);
+    }
+    if (threadIdx.x == 0) {
+        out[0] = (r[0] * s1) * s2;
+        out[1] = ((0.5 * r[1]) * s1) * s2;
+        out[2] = (r[2] * s1) * s2;
+    }
+}
+
+constexpr int kMomentBlocks = 1024;
+
+template <typename T>
+int moments(const T* f, int64_t rows, int64_t len, const double* vx, const double* vy, int64_t nvy, double s1,
+            double s2, double* out, void* ws, size_t ws_bytes, hipStream_t s) {
+    VP_RETURN_IF(!f || !vx || !out || !ws, VP_ERR_ARG);
+    VP_RETURN_IF(rows < 1 || len < 1 || (vy && (nvy < 1 || len % nvy != 0)), VP_ERR_SHAPE);
+    VP_RETURN_IF(ws_bytes < static_cast<size_t>(kMomentBlocks) * 3 * sizeof(double2), VP_ERR_WORKSPACE);
+    const int64_t n = rows * len;
+    int blocks = static_cast<int>((n + 255) / 256);
+    if (blocks > kMomentBlocks) blocks = kMomentBlocks;
+    double2* part = static_cast<double2*>(ws);
+    hipLaunchKernelGGL((moments_partial_kernel<T>), dim3(blocks), dim3(256), 0, s, f, rows, len, vx, vy, nvy, part);
+    VP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(256), 0, s, part, blocks, s1, s2, out);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
 }  // namespace vp
 
 extern "C" int vp_row_sum_f64(const double* f, int64_t rows, int64_t len, double scale1, double scale2,
@@ -288,4 +361,18 @@ extern "C" int vp_poisson_2d_f64(const double* rho, int64_t nx, int64_t ny, cons
                                  void* ws, size_t ws_bytes, vp_stream_t stream) {
     return vp::poisson2d(rho, nx, ny, kx, ky, inv_ksq, Ex, Ey, delta_rho, phi, ws, ws_bytes,
                          static_cast<hipStream_t>(stream));
+}
+
+extern "C" size_t vp_moments_workspace(void) { return static_cast<size_t>(vp::kMomentBlocks) * 3 * sizeof(double2); }
+
+extern "C" int vp_moments_f64(const double* f, int64_t rows, int64_t len, const double* vx, const double* vy,
+                              int64_t nvy, double s1, double s2, double* out, void* ws, size_t ws_bytes,
+                              vp_stream_t stream) {
+    return vp::moments<double>(f, rows, len, vx, vy, nvy, s1, s2, out, ws, ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_moments_f32(const float* f, int64_t rows, int64_t len, const double* vx, const double* vy,
+                              int64_t nvy, double s1, double s2, double* out, void* ws, size_t ws_bytes,
+                              vp_stream_t stream) {
+    return vp::moments<float>(f, rows, len, vx, vy, nvy, s1, s2, out, ws, ws_bytes, static_cast<hipStream_t>(stream));
 }

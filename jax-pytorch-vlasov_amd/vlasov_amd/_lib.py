@@ -62,6 +62,9 @@ _SIGS = {
                                      c_sz, c_vp]),
     "vp_fused_kick_f32": (c_int, [c_vp, c_vp, I64x4, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp]),
     "vp_field_energy_f64": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
+    "vp_moments_workspace": (c_sz, []),
+    "vp_moments_f64": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_sz, c_vp]),
+    "vp_moments_f32": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_sz, c_vp]),
     "vp_ens_workspace": (c_sz, [c_int, c_int, c_int]),
     "vp_ens_run_f64": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_int, c_vp, c_vp, c_sz, c_vp]),
     "vp_nn_gemm": (c_int, [c_int, c_int, c_int, c_int, ctypes.c_float, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64,

@@ -286,6 +286,29 @@ def step_with_field(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, grid_dx, grid_dv
     return out, rho, phi, E
 
 
+def field_energy(E, dx) -> torch.Tensor:
+    """W_E = 1/2 sum E^2 dx (compensated), a [1] float64 device tensor."""
+    dev = require_gpu(E)
+    out = torch.empty(1, dtype=torch.float64, device=dev)
+    check(lib().vp_field_energy_f64(_f64(E).contiguous().data_ptr(), None, E.numel(), float(dx), 1.0,
+                                    out.data_ptr(), stream_handle(dev)), "vp_field_energy_f64")
+    return out
+
+
+def moments(f, grid) -> torch.Tensor:
+    """(mass, kinetic energy, L2 norm squared) of f [Nx, Nv]:
+    sum f dx dv, 1/2 sum f v^2 dx dv, sum f^2 dx dv (compensated), [3] float64."""
+    dev = require_gpu(f)
+    f_c = f.contiguous()
+    Nx, Nv = f_c.shape
+    out = torch.empty(3, dtype=torch.float64, device=dev)
+    ws = _lib.Workspace.get(dev, lib().vp_moments_workspace(), "moments")
+    fn = lib().vp_moments_f64 if f_c.dtype == torch.float64 else lib().vp_moments_f32
+    check(fn(f_c.data_ptr(), Nx, Nv, grid.v.to(torch.float64).contiguous().data_ptr(), None, 0, float(grid.dx),
+             float(grid.dv), out.data_ptr(), ws.data_ptr(), ws.numel(), stream_handle(dev)), "vp_moments")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Driver (P1:757-994)
 # ---------------------------------------------------------------------------
@@ -324,7 +347,8 @@ class Vlasov:
         self.solver = solver
         self.solver_type = solver_names[solver]
         self._step_fn = vlasov_step
-        self.energy_history: list = []
+        self.energy_history = np.zeros(0)
+        self.moment_history: list = []  # (step, t, mass, kinetic, L2) at every snapshot
         pathlib.Path(self.out_dir).mkdir(parents=True, exist_ok=True)
         num_steps = self.nbiter // self.diag_steps
         message = f"PyTorch version: {torch.__version__}\n"
@@ -359,25 +383,30 @@ class Vlasov:
         return state
 
     def save_diag(self, state: SimulationState) -> None:
-        """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray)."""
+        """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray),
+        with the moments (mass, kinetic energy, L2) of f."""
+        mom = moments(state.variables.fn, self.grid).cpu().numpy()
+        self.moment_history.append((int(state.step), float(state.t), *[float(m) for m in mom]))
         filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
         np.savez(filename, f=state.variables.fn.detach().cpu().numpy(),
                  f0=state.variables.f0.detach().cpu().numpy(),
                  rho=state.field.rho.detach().cpu().numpy(), phi=state.field.phi.detach().cpu().numpy(),
                  time=np.array([float(state.t)]), x=self.grid.x.cpu().numpy(), v=self.grid.v.cpu().numpy(),
-                 step=int(state.step), dt=self.dt, physics_mode=int(self.physics_mode))
+                 step=int(state.step), dt=self.dt, physics_mode=int(self.physics_mode), moments=mom)
 
     def run(self, state: SimulationState) -> SimulationState:
-        """Time loop of P1:943-994."""
+        """Time loop of P1:943-994.  ``self.energy_history`` receives the field
+        energy 1/2 sum E^2 dx of every step's mid-step field."""
         g = self.grid
         diag_counter = 0
         rho, phi = state.field.rho, state.field.phi
         f = state.variables.fn
+        energy = torch.zeros(max(self.nbiter, 1), dtype=torch.float64, device=self.device)
         synchronize(self.device)
-        for _ in range(1, self.nbiter + 1):
-            f, rho, phi = self._step_fn(f, self.dt, g.v, g.inv_kx, g.inv_kx_sq, g.x_indices, g.v_indices,
-                                        g.x_batch_offsets, g.v_batch_offsets, g.stencil_offsets, g.dx, g.dv,
-                                        g.V_max, self.physics_mode)
+        for i in range(self.nbiter):
+            f, rho, phi, E = step_with_field(f, self.dt, g.v, g.inv_kx, g.inv_kx_sq, g.dx, g.dv, self.physics_mode)
+            check(lib().vp_field_energy_f64(E.data_ptr(), None, g.Nx, g.dx, 1.0, energy.data_ptr() + 8 * i,
+                                            stream_handle(self.device)), "vp_field_energy_f64")
             state.t += self.dt
             state.step += 1
             diag_counter += 1
@@ -388,5 +417,21 @@ class Vlasov:
                                                t=state.t, step=state.step))
                 diag_counter = 0
         synchronize(self.device)
+        self.energy_history = energy[:self.nbiter].cpu().numpy()
         return SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=state.variables.f0),
                                t=state.t, step=state.step)
+
+
+def main(argv=None) -> str:
+    """The reference script's command line (P1 __main__): same flags, log
+    lines and log file ``vlp1d_1v_<dtype>.txt``."""
+    from . import cli
+    a = cli.parse(argv, two_d=False)
+    solver = Vlasov(Nx=a.nx, Nv=a.nv, L=a.lx, V_max=a.vmax, nbiter=a.nbiter, diag_steps=a.diag_steps, dt=a.dt,
+                    out_dir=a.out_dir, physics_mode=a.physics_mode, epsilon=a.epsilon, dtype=a.dtype,
+                    solver=a.solver)
+    return cli.run(solver, f"vlp1d_1v_{a.dtype}.txt", (get_memory_stats, reset_peak_memory, format_memory))
+
+
+if __name__ == "__main__":
+    main()

@@ -216,10 +216,22 @@ def solve_poisson(f_data, kx, ky, inv_ksq, dvx, dvy):
 
 def step_with_field(f, dt, grid_vx, grid_vy, grid_kx, grid_ky, grid_inv_ksq, grid_dx, grid_dy, grid_dvx,
                     grid_dvy, physics_mode, out=None):
-    """One Strang step; returns (f, delta_rho, phi, Ex, Ey)."""
+    """One Strang step; returns (f, delta_rho, phi, Ex, Ey).  float32 f runs
+    the FP32-storage variant (fields stay float64)."""
     dev = require_gpu(f, grid_vx, grid_vy)
+    if f.dtype == torch.float32:
+        # FP32 storage: one step of the in-place runner on a copy of f
+        from types import SimpleNamespace
+
+        from .runner import Run2D
+        g = SimpleNamespace(vx=grid_vx, vy=grid_vy, kx=grid_kx, ky=grid_ky, inv_ksq=grid_inv_ksq, dx=float(grid_dx),
+                            dy=float(grid_dy), dvx=float(grid_dvx), dvy=float(grid_dvy))
+        run = Run2D(g, f, dt, bool(physics_mode))
+        run.advance(1)
+        res = run.state if out is None else out.copy_(run.state)
+        return res, run.delta_rho.clone(), run.phi.clone(), run.Ex.clone(), run.Ey.clone()
     if f.dtype != torch.float64:
-        raise TypeError("step_with_field runs float64")
+        raise TypeError(f"f must be float64 or float32 (FP32 storage), got {f.dtype}")
     f_c = f.contiguous()
     Nx, Ny = f_c.shape[0], f_c.shape[1]
     if out is None:
@@ -251,6 +263,23 @@ def vlasov_step_impl(f, f0, dt, grid_x, grid_y, grid_vx, grid_vy, grid_kx, grid_
     return f, rho, phi
 
 
+def moments(f, grid) -> torch.Tensor:
+    """(mass, kinetic energy, L2 norm squared) of f [Nx, Ny, Nvx, Nvy]:
+    sum f, 1/2 sum f (vx^2 + vy^2), sum f^2, each times dx dy dvx dvy
+    (compensated float64 sums), a [3] float64 device tensor."""
+    dev = require_gpu(f)
+    f_c = f.contiguous()
+    Nx, Ny, Nvx, Nvy = f_c.shape
+    out = torch.empty(3, dtype=torch.float64, device=dev)
+    ws = _lib.Workspace.get(dev, lib().vp_moments_workspace(), "moments")
+    fn = lib().vp_moments_f64 if f_c.dtype == torch.float64 else lib().vp_moments_f32
+    cell = grid.dx * grid.dy * grid.dvx * grid.dvy
+    check(fn(f_c.data_ptr(), Nx * Ny, Nvx * Nvy, grid.vx.to(torch.float64).contiguous().data_ptr(),
+             grid.vy.to(torch.float64).contiguous().data_ptr(), Nvy, float(cell), 1.0, out.data_ptr(), ws.data_ptr(),
+             ws.numel(), stream_handle(dev)), "vp_moments")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Driver (P2:1223-1435)
 # ---------------------------------------------------------------------------
@@ -270,10 +299,14 @@ class Vlasov:
 
         ``solver`` keeps the reference ids for the log; every id runs the HIP
         step, which follows solver 0's arithmetic (vlasov_step_impl).
+        dtype='float32' is the FP32-storage variant: f is float32, the grid,
+        charge density, Poisson solve and fields stay float64 (the
+        reference's float32 mode computes everything in float32).
         """
         self.output_log = ""
-        if dtype != "float64":
-            raise ValueError("vlasov_amd.vlasov2D2V.Vlasov runs float64 (dtype='float64')")
+        if dtype not in ("float64", "float32"):
+            raise ValueError(f"dtype must be 'float64' or 'float32', got {dtype!r}")
+        self.f_dtype = torch.float64 if dtype == "float64" else torch.float32
         self.dtype = torch.float64
         self.device = torch.device(device) if device is not None else default_device()
         self.grid = Grid.create(Nx, Ny, Nvx, Nvy, Lx, Ly, Vx_max, Vy_max, device=self.device, dtype=self.dtype)
@@ -287,10 +320,12 @@ class Vlasov:
         self.solver = solver
         self.solver_type = solver_names[solver]
         self._step_fn = vlasov_step_impl
+        self.energy_history = np.zeros(0)
+        self.moment_history: list = []  # (step, t, mass, kinetic, L2) at every snapshot
         pathlib.Path(self.out_dir).mkdir(parents=True, exist_ok=True)
         num_steps = self.nbiter // self.diag_steps
         message = f"PyTorch version: {torch.__version__}\n"
-        message += f"Starting simulation on {self.device} with {self.grid.x.dtype} precision "
+        message += f"Starting simulation on {self.device} with {self.f_dtype} precision "
         message += "with Poisson solver" if self.physics_mode else "without Poisson solver"
         message += f" and solver type {self.solver_type}."
         self._append_log(message)
@@ -324,77 +359,79 @@ class Vlasov:
             vy2 = (vy_mesh[None, None, :, :] - 0.0) ** 2 / sigma_v ** 2
             f_data = torch.exp(-0.5 * (dx2 + dy2 + vx2 + vy2))
             f0 = torch.zeros((g.Nvx, g.Nvy), device=self.device, dtype=self.dtype)
+        f_data = f_data.to(self.f_dtype).contiguous()
         *_, rho, phi = solve_poisson_real(f_data, g.kx, g.ky, g.inv_ksq, g.dvx, g.dvy)
         state = SimulationState(field=Field(rho, phi), variables=Variables(fn=f_data, f0=f0), t=0.0, step=0)
         self.save_diag(state)
         return state
 
     def save_diag(self, state: SimulationState) -> None:
-        """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray)."""
+        """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray),
+        with the moments (mass, kinetic energy, L2) of f."""
         g = self.grid
+        mom = moments(state.variables.fn, g).cpu().numpy()
+        self.moment_history.append((int(state.step), float(state.t), *[float(m) for m in mom]))
         filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
         np.savez(filename, f=state.variables.fn.detach().cpu().numpy(), f0=state.variables.f0.detach().cpu().numpy(),
                  rho=state.field.rho.detach().cpu().numpy(), phi=state.field.phi.detach().cpu().numpy(),
                  time=np.array([float(state.t)]), x=g.x.cpu().numpy(), y=g.y.cpu().numpy(),
                  vx=g.vx.cpu().numpy(), vy=g.vy.cpu().numpy(), step=int(state.step), dt=self.dt,
-                 Lx=g.Lx, Ly=g.Ly, Vx_max=g.Vx_max, Vy_max=g.Vy_max, physics_mode=int(self.physics_mode))
+                 Lx=g.Lx, Ly=g.Ly, Vx_max=g.Vx_max, Vy_max=g.Vy_max, physics_mode=int(self.physics_mode),
+                 moments=mom)
 
     def run(self, state: SimulationState) -> SimulationState:
         """Time loop of P2:1392-1435.
 
         Same steps, diagnostics cadence and returned state as the reference.
-        For shapes with every extent in {64, 128} the steps between two
-        diagnostics run in place through the fused kernels (``runner.Run2D``),
-        otherwise step by step.  ``self.energy_history`` receives the field
-        energy 1/2 sum(Ex^2+Ey^2) dx dy of every step (mid-step field).
+        The steps between two snapshots run in place through
+        ``runner.Run2D`` (fused kernels where the shape has them, else the
+        generic sweeps; both equal repeated vlasov_step_impl calls).
+        ``self.energy_history`` receives the field energy
+        1/2 sum(Ex^2+Ey^2) dx dy of every step (mid-step field).
         """
-        from .runner import Run2D, fused_supported
+        from .runner import Run2D
 
         g = self.grid
         rho, phi = state.field.rho, state.field.phi
         f = state.variables.fn
         energy = torch.zeros(max(self.nbiter, 1), dtype=torch.float64, device=self.device)
         synchronize(self.device)
-        if fused_supported(f.shape) and f.dtype == torch.float64:
-            runner = Run2D(g, f, self.dt, self.physics_mode, fused=True)
-            done = 0
-            while done < self.nbiter:
-                chunk = min(self.diag_steps, self.nbiter - done)
-                runner.advance(chunk, energy=energy[done:done + chunk])
-                done += chunk
-                for _ in range(chunk):
-                    state.t += self.dt
-                state.step += chunk
-                if chunk == self.diag_steps:
-                    synchronize(self.device)
-                    self.save_diag(SimulationState(field=Field(runner.delta_rho.clone(), runner.phi.clone()),
-                                                   variables=Variables(fn=runner.state, f0=state.variables.f0),
-                                                   t=state.t, step=state.step))
-            f = runner.state
-            if self.nbiter > 0:
-                rho, phi = runner.delta_rho.clone(), runner.phi.clone()
-        else:
-            diag_counter = 0
-            for i in range(self.nbiter):
-                f, rho, phi, Ex, Ey = step_with_field(f, self.dt, g.vx, g.vy, g.kx, g.ky, g.inv_ksq, g.dx, g.dy,
-                                                      g.dvx, g.dvy, self.physics_mode)
-                check(lib().vp_field_energy_f64(Ex.data_ptr(), Ey.data_ptr(), g.Nx * g.Ny, g.dx, g.dy,
-                                                energy.data_ptr() + 8 * i, stream_handle(self.device)),
-                      "vp_field_energy_f64")
+        runner = Run2D(g, f, self.dt, self.physics_mode)
+        done = 0
+        while done < self.nbiter:
+            chunk = min(self.diag_steps, self.nbiter - done)
+            runner.advance(chunk, energy=energy[done:done + chunk])
+            done += chunk
+            for _ in range(chunk):
                 state.t += self.dt
-                state.step += 1
-                diag_counter += 1
-                if diag_counter >= self.diag_steps:
-                    synchronize(self.device)
-                    self.save_diag(SimulationState(field=Field(rho, phi),
-                                                   variables=Variables(fn=f, f0=state.variables.f0),
-                                                   t=state.t, step=state.step))
-                    diag_counter = 0
+            state.step += chunk
+            if chunk == self.diag_steps:
+                synchronize(self.device)
+                self.save_diag(SimulationState(field=Field(runner.delta_rho.clone(), runner.phi.clone()),
+                                               variables=Variables(fn=runner.state, f0=state.variables.f0),
+                                               t=state.t, step=state.step))
+        f = runner.state
+        if self.nbiter > 0:
+            rho, phi = runner.delta_rho.clone(), runner.phi.clone()
         synchronize(self.device)
         self.energy_history = energy[:self.nbiter].cpu().numpy()
         return SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=state.variables.f0),
                                t=state.t, step=state.step)
 
 
+def main(argv=None) -> str:
+    """The reference script's command line (P2 __main__): same flags, log
+    lines and log file ``vlp2d_2v_<dtype>.txt``."""
+    from . import cli
+    a = cli.parse(argv, two_d=True)
+    solver = Vlasov(Nx=a.nx, Ny=a.ny, Nvx=a.nvx, Nvy=a.nvy, Lx=a.lx, Ly=a.ly, Vx_max=a.Vx_max, Vy_max=a.Vy_max,
+                    nbiter=a.nbiter, diag_steps=a.diag_steps, dt=a.dt, out_dir=a.out_dir,
+                    physics_mode=a.physics_mode, epsilon=a.epsilon, dtype=a.dtype, solver=a.solver)
+    return cli.run(solver, f"vlp2d_2v_{a.dtype}.txt", (get_memory_stats, reset_peak_memory, format_memory))
+
+
 __all__ = [name for name in dir() if not name.startswith("_")]
 _ = format_memory
+
+if __name__ == "__main__":
+    main()
