@@ -69,6 +69,27 @@ int vp_advect_f32(const float* f_in, float* f_out, const int64_t shape[4], int a
                   const double* vel, const int64_t vel_strides[4],
                   double dt, double dx, int periodic, vp_stream_t stream);
 
+/* Reflected (shift) form of the sweep, used by the reference's solver 1
+ * (advect_1d_matrix_diag, P1:283-320, P2:560-641) and solver 2
+ * (advect_v_static, P1:479-501, P2:868-913): d = (vel*dt)/dx,
+ * s = floor(d), alpha = d - s, out[i] = sum_k w_k(alpha) in[i - s - o_k]
+ * with o_k = k - 2 (taps in that order, wrapped or zero outside).  Agrees
+ * with vp_advect_f64 to roundoff (the reference's tests allow 1e-10).
+ * window >= 0 (Dirichlet only) keeps just the taps with |s + o_k| <=
+ * window + 2, the static window of solver 2 (exact while |d| < window);
+ * window < 0 keeps all taps (solver 1). */
+int vp_advect_reflected_f64(const double* f_in, double* f_out, const int64_t shape[4], int axis,
+                            const double* vel, const int64_t vel_strides[4], double dt, double dx,
+                            int periodic, int window, vp_stream_t stream);
+
+/* advect_x_static (P1:408-448, P2:756-819): periodic sweep with the shift
+ * and the six weights of every line given: out[i] = sum_k W[k][p]
+ * in[(i - shift[p] - o_k) mod n], p = sum_d i_d * param_strides[d] (the
+ * line's parameter index), W[k][p] at W[k * w_tap_stride + p]. */
+int vp_advect_static_f64(const double* f_in, double* f_out, const int64_t shape[4], int axis,
+                         const int64_t* shift, const double* W, int64_t w_tap_stride,
+                         const int64_t param_strides[4], vp_stream_t stream);
+
 /* ---------------------------------------------------------------------
  * Velocity moments and field solves.
  * --------------------------------------------------------------------- */

@@ -65,11 +65,68 @@ __device__ __forceinline__ T interp_from_lds(const T* tile, int64_t il, double a
     return tap_sum<T>(w, v);
 }
 
+// Sweep forms.
+//   kDeparture: the reference sweep (advect_1d_vectorized, vlasov1D1V.py:212-256):
+//     dep = i - (v dt)/dx, il = floor(dep), alpha = dep - il, taps il - 2 .. il + 3.
+//   kReflected: the roll/shift forms of solvers 1 and 2 (advect_1d_matrix_diag,
+//     vlasov1D1V.py:283-320; advect_v_static, vlasov1D1V.py:479-501):
+//     d = (v dt)/dx, s = floor(d), alpha = d - s, tap k (offset o = k - 2) reads
+//     i - s - o with weight w_k(alpha); `window` >= 0 keeps only taps with
+//     |s + o| <= window + 2 (the static window of solver 2), < 0 keeps all.
+//   kStatic: kReflected with the shift s and the six weights given per line
+//     (advect_x_static, vlasov1D1V.py:408-448): periodic only.
+enum SweepForm { kDeparture = 0, kReflected = 1, kStatic = 2 };
+
+struct SweepParams {
+    const double* vel;     // kDeparture, kReflected
+    const int64_t* shift;  // kStatic
+    const double* W;       // kStatic: W[k * wk + offset]
+    int64_t wk;
+    double dt, dx;
+    int window;
+};
+
+template <typename T, bool PERIODIC, int FORM>
+__device__ __forceinline__ T sweep_value(const T* tile, int r, int64_t poff, const SweepParams& P, int n,
+                                         int row_stride, int col) {
+    if (FORM == kDeparture) {
+        const Departure d = departure(r, P.vel[poff], P.dt, P.dx);
+        return interp_from_lds<T, PERIODIC>(tile, d.il, d.alpha, n, row_stride, col);
+    }
+    int64_t sh;
+    T w[6];
+    if (FORM == kReflected) {
+        const double disp = (P.vel[poff] * P.dt) / P.dx;
+        const double fl = floor(disp);
+        weights_as<T>(disp - fl, w);
+        sh = static_cast<int64_t>(fl);
+    } else {
+        sh = P.shift[poff];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = static_cast<T>(P.W[k * P.wk + poff]);
+    }
+    T v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int64_t j = r - sh - (k - 2);
+        if (PERIODIC) {
+            v[k] = tile[wrap_index(j, n) * row_stride + col];
+        } else {
+            const int jc = static_cast<int>(j < 0 ? 0 : (j > n - 1 ? n - 1 : j));
+            const int64_t m = sh + (k - 2);
+            const bool in_window = P.window < 0 || (m >= -(P.window + 2) && m <= P.window + 2);
+            const T valid = (j >= 0 && j < n && in_window) ? T(1) : T(0);
+            v[k] = tile[jc * row_stride + col] * valid;
+        }
+    }
+    return tap_sum<T>(w, v);
+}
+
 // Tile = n rows (the swept axis) x C columns (contiguous inner elements).
-template <typename T, bool PERIODIC>
+template <typename T, bool PERIODIC, int FORM>
 __global__ __launch_bounds__(256) void advect_strided_kernel(
     const T* __restrict__ fin, T* __restrict__ fout, int n, int64_t inner, int C,
-    int64_t n_chunks, const double* __restrict__ vel, VelMap vm, double dt, double dx) {
+    int64_t n_chunks, SweepParams P, VelMap vm) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* tile = reinterpret_cast<T*>(smem_raw);
     const int64_t blk = blockIdx.x;
@@ -85,17 +142,14 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
     __syncthreads();
     if (!cval) return;
     const int64_t voff = split_offset(o, vm.od, vm.os, vm.no) + split_offset(i0 + c, vm.id, vm.is, vm.ni);
-    for (int r = r0; r < n; r += rstep) {
-        const Departure d = departure(r, vel[voff + r * vm.s_line], dt, dx);
-        fout[base + r * inner] = interp_from_lds<T, PERIODIC>(tile, d.il, d.alpha, n, C, c);
-    }
+    for (int r = r0; r < n; r += rstep)
+        fout[base + r * inner] = sweep_value<T, PERIODIC, FORM>(tile, r, voff + r * vm.s_line, P, n, C, c);
 }
 
 // Tile = L whole contiguous lines of length n; wave w handles lines w, w+4, ...
-template <typename T, bool PERIODIC>
+template <typename T, bool PERIODIC, int FORM>
 __global__ __launch_bounds__(256) void advect_contig_kernel(
-    const T* __restrict__ fin, T* __restrict__ fout, int64_t n_lines, int n, int L,
-    const double* __restrict__ vel, VelMap vm, double dt, double dx) {
+    const T* __restrict__ fin, T* __restrict__ fout, int64_t n_lines, int n, int L, SweepParams P, VelMap vm) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* tile = reinterpret_cast<T*>(smem_raw);
     const int64_t l0 = static_cast<int64_t>(blockIdx.x) * L;
@@ -111,17 +165,15 @@ __global__ __launch_bounds__(256) void advect_contig_kernel(
         const int64_t voff = split_offset(l0 + l, vm.od, vm.os, vm.no);
         const T* line = tile + static_cast<int64_t>(l) * n;
         T* dst = fout + (l0 + l) * n;
-        for (int j = lane; j < n; j += kWave) {
-            const Departure d = departure(j, vel[voff + j * vm.s_line], dt, dx);
-            dst[j] = interp_from_lds<T, PERIODIC>(line, d.il, d.alpha, n, 1, 0);
-        }
+        for (int j = lane; j < n; j += kWave)
+            dst[j] = sweep_value<T, PERIODIC, FORM>(line, j, voff + j * vm.s_line, P, n, 1, 0);
     }
 }
 
-template <typename T>
-static int launch_advect(const T* fin, T* fout, const int64_t shape[4], int axis, const double* vel,
-                         const int64_t vs[4], double dt, double dx, int periodic, hipStream_t s) {
-    VP_RETURN_IF(!fin || !fout || !shape || !vel || !vs, VP_ERR_ARG);
+template <typename T, int FORM>
+static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis, const SweepParams& P,
+                        const int64_t vs[4], int periodic, hipStream_t s) {
+    VP_RETURN_IF(!fin || !fout || !shape || !vs, VP_ERR_ARG);
     VP_RETURN_IF(axis < 0 || axis > 3, VP_ERR_ARG);
     for (int d = 0; d < 4; ++d) VP_RETURN_IF(shape[d] < 1, VP_ERR_SHAPE);
     const int64_t n64 = shape[axis];
@@ -153,11 +205,11 @@ static int launch_advect(const T* fin, T* fout, const int64_t shape[4], int axis
         VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
         const size_t lds = static_cast<size_t>(n) * C * sizeof(T);
         if (periodic)
-            hipLaunchKernelGGL((advect_strided_kernel<T, true>), dim3(blocks), dim3(block), lds, s, fin,
-                               fout, n, inner, C, chunks, vel, vm, dt, dx);
+            hipLaunchKernelGGL((advect_strided_kernel<T, true, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
+                               n, inner, C, chunks, P, vm);
         else
-            hipLaunchKernelGGL((advect_strided_kernel<T, false>), dim3(blocks), dim3(block), lds, s, fin,
-                               fout, n, inner, C, chunks, vel, vm, dt, dx);
+            hipLaunchKernelGGL((advect_strided_kernel<T, false, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
+                               n, inner, C, chunks, P, vm);
     } else {
         int L = n >= 4096 ? 1 : static_cast<int>(4096 / n);
         if (L > outer) L = static_cast<int>(outer);
@@ -165,14 +217,22 @@ static int launch_advect(const T* fin, T* fout, const int64_t shape[4], int axis
         VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
         const size_t lds = static_cast<size_t>(L) * n * sizeof(T);
         if (periodic)
-            hipLaunchKernelGGL((advect_contig_kernel<T, true>), dim3(blocks), dim3(block), lds, s, fin, fout,
-                               outer, n, L, vel, vm, dt, dx);
+            hipLaunchKernelGGL((advect_contig_kernel<T, true, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
+                               outer, n, L, P, vm);
         else
-            hipLaunchKernelGGL((advect_contig_kernel<T, false>), dim3(blocks), dim3(block), lds, s, fin, fout,
-                               outer, n, L, vel, vm, dt, dx);
+            hipLaunchKernelGGL((advect_contig_kernel<T, false, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
+                               outer, n, L, P, vm);
     }
     VP_CHECK_LAUNCH();
     return VP_OK;
+}
+
+template <typename T>
+static int launch_advect(const T* fin, T* fout, const int64_t shape[4], int axis, const double* vel,
+                         const int64_t vs[4], double dt, double dx, int periodic, hipStream_t s) {
+    VP_RETURN_IF(!vel, VP_ERR_ARG);
+    SweepParams P{vel, nullptr, nullptr, 0, dt, dx, -1};
+    return launch_sweep<T, kDeparture>(fin, fout, shape, axis, P, vs, periodic, s);
 }
 
 }  // namespace vp
@@ -189,4 +249,23 @@ extern "C" int vp_advect_f32(const float* f_in, float* f_out, const int64_t shap
                              int periodic, vp_stream_t stream) {
     return vp::launch_advect<float>(f_in, f_out, shape, axis, vel, vel_strides, dt, dx, periodic,
                                     static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_advect_reflected_f64(const double* f_in, double* f_out, const int64_t shape[4], int axis,
+                                       const double* vel, const int64_t vel_strides[4], double dt, double dx,
+                                       int periodic, int window, vp_stream_t stream) {
+    VP_RETURN_IF(!vel, VP_ERR_ARG);
+    VP_RETURN_IF(periodic && window >= 0, VP_ERR_ARG);
+    vp::SweepParams P{vel, nullptr, nullptr, 0, dt, dx, window};
+    return vp::launch_sweep<double, vp::kReflected>(f_in, f_out, shape, axis, P, vel_strides, periodic,
+                                                    static_cast<hipStream_t>(stream));
+}
+
+extern "C" int vp_advect_static_f64(const double* f_in, double* f_out, const int64_t shape[4], int axis,
+                                    const int64_t* shift, const double* W, int64_t w_tap_stride,
+                                    const int64_t param_strides[4], vp_stream_t stream) {
+    VP_RETURN_IF(!shift || !W, VP_ERR_ARG);
+    vp::SweepParams P{nullptr, shift, W, w_tap_stride, 0.0, 1.0, -1};
+    return vp::launch_sweep<double, vp::kStatic>(f_in, f_out, shape, axis, P, param_strides, 1,
+                                                 static_cast<hipStream_t>(stream));
 }

@@ -286,6 +286,100 @@ def step_with_field(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, grid_dx, grid_dv
     return out, rho, phi, E
 
 
+# ---------------------------------------------------------------------------
+# Solver 1 and solver 2 (shift forms, P1:258-512, P1:625-755)
+# ---------------------------------------------------------------------------
+from . import shift_forms as _sf  # noqa: E402
+
+static_stencil_offsets = _sf.static_stencil_offsets
+precompute_x_drift = _sf.precompute_x_drift
+build_v_tap_weights = _sf.build_v_tap_weights
+check_v_window = _sf.check_v_window
+
+
+def build_v_diag_weights(stencil_offsets, shifts, alpha, n_points):
+    """[6, Nx, Nv] banded weights of the solver-1 V sweep (P1:280-300)."""
+    return _sf.build_diag_weights(stencil_offsets, shifts, alpha, n_points)
+
+
+def advect_1d_matrix_diag(f_batch, velocities, dt, dx, stencil_offsets, periodic=True):
+    """Drop-in for advect_1d_matrix_diag (P1:302-337): periodic -> sweep
+    along dim 0 with one velocity per column ([Nv]); else along dim 1 with
+    one velocity per row ([Nx]), zero outside."""
+    _sf._check_offsets(stencil_offsets)
+    Nx, Nv = f_batch.shape
+    f4 = f_batch.reshape(1, 1, Nx, Nv)
+    vel = velocities.reshape(-1)
+    if periodic:
+        out = _sf.sweep_reflected(f4, 2, vel, (0, 0, 0, 1), dt, dx, True)
+    else:
+        out = _sf.sweep_reflected(f4, 3, vel, (0, 0, 1, 0), dt, dx, False)
+    return out.reshape(Nx, Nv)
+
+
+def block_roll(f, blocks):
+    """Roll each column block by its static shift along dim 0 (P1:389-406)."""
+    Nx, Nv = f.shape
+    if all(shift % Nx == 0 for _, _, shift in blocks):
+        return f
+    out = _sf.static_sweep(f.reshape(1, 1, Nx, Nv), 2, _sf.block_shifts(blocks, f.device),
+                           _sf.roll_weights(Nv, f.device), (0, 0, 0, 1))
+    return out.reshape(Nx, Nv)
+
+
+def advect_x_static(f, W_x, blocks, stencil_offsets):
+    """Drop-in for advect_x_static (P1:408-448): periodic X drift with the
+    static shifts of `blocks` and the weights W_x [6, Nv]."""
+    _sf._check_offsets(stencil_offsets)
+    Nx, Nv = f.shape
+    out = _sf.static_sweep(f.reshape(1, 1, Nx, Nv), 2, _sf.block_shifts(blocks, f.device), W_x, (0, 0, 0, 1))
+    return out.reshape(Nx, Nv)
+
+
+def advect_v_static(f, E, dt, dv, S, stencil_offsets):
+    """Drop-in for advect_v_static (P1:479-501): Dirichlet V kick exact while
+    |E dt/dv| < S (taps outside the static window are dropped, as there)."""
+    _sf._check_offsets(stencil_offsets)
+    Nx, Nv = f.shape
+    out = _sf.sweep_reflected(f.reshape(1, 1, Nx, Nv), 3, E.reshape(-1), (0, 0, 1, 0), dt, dv, False, int(S))
+    return out.reshape(Nx, Nv)
+
+
+def _field(f, physics_mode, inv_kx, inv_kx_sq, dv):
+    if physics_mode:
+        return solve_poisson_real(f, inv_kx, inv_kx_sq, dv)
+    z = f.new_zeros((f.shape[0],))
+    return z, z.clone(), z.clone()
+
+
+def vlasov_step_matrix_diag(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, x_indices, v_indices, x_batch_offsets,
+                            v_batch_offsets, stencil_offsets, grid_dx, grid_dv, grid_V_max, physics_mode):
+    """Drop-in for vlasov_step_matrix_diag (P1:625-665), the solver-1 step."""
+    f = advect_1d_matrix_diag(f, grid_v, dt / 2.0, grid_dx, stencil_offsets, periodic=True)
+    E, rho, phi = _field(f, physics_mode, grid_inv_kx, grid_inv_kx_sq, grid_dv)
+    f = advect_1d_matrix_diag(f, E, dt, grid_dv, stencil_offsets, periodic=False)
+    f = advect_1d_matrix_diag(f, grid_v, dt / 2.0, grid_dx, stencil_offsets, periodic=True)
+    return f, rho, phi
+
+
+def vlasov_step_gather_free_impl(f, dt, W_x, blocks, stencil_offsets, grid_inv_kx, grid_inv_kx_sq, grid_dv,
+                                 physics_mode, S):
+    """Drop-in for vlasov_step_gather_free_impl (P1:668-701), the solver-2 core."""
+    f = advect_x_static(f, W_x, blocks, stencil_offsets)
+    E, rho, phi = _field(f, physics_mode, grid_inv_kx, grid_inv_kx_sq, grid_dv)
+    f = advect_v_static(f, E, dt, grid_dv, S, stencil_offsets)
+    f = advect_x_static(f, W_x, blocks, stencil_offsets)
+    return f, rho, phi
+
+
+def vlasov_step_gather_free(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, x_indices, v_indices, x_batch_offsets,
+                            v_batch_offsets, stencil_offsets, grid_dx, grid_dv, grid_V_max, physics_mode, S=1):
+    """Drop-in for vlasov_step_gather_free (P1:703-755): precompute, then the core."""
+    blocks, W_x = precompute_x_drift(grid_v, dt / 2.0, grid_dx, f.dtype)
+    return vlasov_step_gather_free_impl(f, dt, W_x, blocks, static_stencil_offsets(stencil_offsets), grid_inv_kx,
+                                        grid_inv_kx_sq, grid_dv, physics_mode, S)
+
+
 def field_energy(E, dx) -> torch.Tensor:
     """W_E = 1/2 sum E^2 dx (compensated), a [1] float64 device tensor."""
     dev = require_gpu(E)
@@ -327,9 +421,10 @@ class Vlasov:
                  solver: int = 0, device=None) -> None:
         """Same parameters as the reference constructor (P1:770-876).
 
-        ``solver`` keeps the reference ids for log compatibility; every id
-        runs the HIP step, which follows solver 0's arithmetic
-        (vlasov_step).  Only float64 is supported by this class.
+        ``solver`` selects the step as in the reference: 0 vlasov_step,
+        1 vlasov_step_matrix_diag, 2 vlasov_step_gather_free (static shift
+        data hoisted out of the loop).  Only float64 is supported by this
+        class.
         """
         self.output_log = ""
         if dtype != "float64":
@@ -343,10 +438,24 @@ class Vlasov:
         self.out_dir = out_dir
         self.physics_mode = physics_mode
         self.epsilon = epsilon
+        solver_fns = {0: vlasov_step, 1: vlasov_step_matrix_diag, 2: vlasov_step_gather_free}
         solver_names = {0: "vlasov_step", 1: "vlasov_step_matrix_diag", 2: "vlasov_step_gather_free"}
         self.solver = solver
         self.solver_type = solver_names[solver]
-        self._step_fn = vlasov_step
+        self._step_fn = solver_fns[solver]
+        if solver == 2:
+            # static drift data hoisted out of the loop, as in P1:843-865
+            g = self.grid
+            blocks, W_x = precompute_x_drift(g.v, self.dt / 2.0, g.dx, self.dtype)
+            offsets = static_stencil_offsets(g.stencil_offsets)
+
+            def _step_gather_free(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, x_indices, v_indices,
+                                  x_batch_offsets, v_batch_offsets, stencil_offsets, grid_dx, grid_dv, grid_V_max,
+                                  physics_mode):
+                return vlasov_step_gather_free_impl(f, dt, W_x, blocks, offsets, grid_inv_kx, grid_inv_kx_sq,
+                                                    grid_dv, physics_mode, 1)
+
+            self._step_fn = _step_gather_free
         self.energy_history = np.zeros(0)
         self.moment_history: list = []  # (step, t, mass, kinetic, L2) at every snapshot
         pathlib.Path(self.out_dir).mkdir(parents=True, exist_ok=True)
@@ -404,7 +513,16 @@ class Vlasov:
         energy = torch.zeros(max(self.nbiter, 1), dtype=torch.float64, device=self.device)
         synchronize(self.device)
         for i in range(self.nbiter):
-            f, rho, phi, E = step_with_field(f, self.dt, g.v, g.inv_kx, g.inv_kx_sq, g.dx, g.dv, self.physics_mode)
+            if self.solver == 0:
+                f, rho, phi, E = step_with_field(f, self.dt, g.v, g.inv_kx, g.inv_kx_sq, g.dx, g.dv,
+                                                 self.physics_mode)
+            else:
+                f, rho, phi = self._step_fn(f, self.dt, g.v, g.inv_kx, g.inv_kx_sq, g.x_indices, g.v_indices,
+                                            g.x_batch_offsets, g.v_batch_offsets, g.stencil_offsets, g.dx, g.dv,
+                                            g.V_max, self.physics_mode)
+                # mid-step field from the returned rho (the step's own Poisson input)
+                E, _ = poisson_from_rho(rho, g.inv_kx, g.inv_kx_sq) if self.physics_mode else (
+                    torch.zeros(g.Nx, dtype=torch.float64, device=self.device), None)
             check(lib().vp_field_energy_f64(E.data_ptr(), None, g.Nx, g.dx, 1.0, energy.data_ptr() + 8 * i,
                                             stream_handle(self.device)), "vp_field_energy_f64")
             state.t += self.dt

@@ -263,6 +263,118 @@ def vlasov_step_impl(f, f0, dt, grid_x, grid_y, grid_vx, grid_vy, grid_kx, grid_
     return f, rho, phi
 
 
+# ---------------------------------------------------------------------------
+# Solver 1 and solver 2 (shift forms, P2:526-941, P2:1001-1132)
+# ---------------------------------------------------------------------------
+from . import shift_forms as _sf  # noqa: E402
+
+static_stencil_offsets = _sf.static_stencil_offsets
+precompute_x_drift = _sf.precompute_x_drift
+build_v_tap_weights = _sf.build_v_tap_weights
+check_v_window = _sf.check_v_window
+build_diag_weights = _sf.build_diag_weights
+
+
+def advect_1d_matrix_diag_core(f_batch, velocities, dt, dx, stencil_offsets, periodic=True):
+    """Drop-in for advect_1d_matrix_diag_core (P2:557-590): rows [B, N], one
+    velocity per row ([B]), sweep along the last axis."""
+    _sf._check_offsets(stencil_offsets)
+    B, N = f_batch.shape
+    vel = velocities.reshape(-1).to(torch.float64).expand(B)
+    out = _sf.sweep_reflected(f_batch.reshape(1, 1, B, N), 3, vel, (0, 0, vel.stride(0), 0), dt, dx, periodic)
+    return out.reshape(B, N)
+
+
+def advect_1d_matrix_diag(f_data, velocities, axis, dt, dx, periodic=True, point_indices=None, stencil_offsets=None):
+    """Drop-in for advect_1d_matrix_diag (P2:592-641): the solver-1 sweep of
+    f(x, y, vx, vy) along ``axis`` (velocities broadcast to the batch
+    shape), without the permute/reshape copies."""
+    _sf._check_offsets(stencil_offsets)
+    axis = int(axis) % 4
+    batch_shape = tuple(n for i, n in enumerate(f_data.shape) if i != axis)
+    vel = velocities.to(device=f_data.device, dtype=torch.float64).expand(batch_shape)
+    strides = list(vel.stride())
+    strides.insert(axis, 0)
+    return _sf.sweep_reflected(f_data, axis, vel, strides, dt, dx, periodic)
+
+
+def block_roll(f_data, blocks, axis, vel_axis):
+    """Roll each velocity block by its static shift along ``axis`` (P2:717-754)."""
+    if all(shift % f_data.shape[axis] == 0 for _, _, shift in blocks):
+        return f_data
+    strides = [0, 0, 0, 0]
+    strides[vel_axis] = 1
+    return _sf.static_sweep(f_data, axis, _sf.block_shifts(blocks, f_data.device),
+                            _sf.roll_weights(f_data.shape[vel_axis], f_data.device), strides)
+
+
+def advect_x_static(f_data, W, blocks, stencil_offsets, axis):
+    """Drop-in for advect_x_static (P2:756-819): periodic drift along axis 0
+    (velocity vx) or 1 (vy) with static shifts and weights W [6, Nv]."""
+    _sf._check_offsets(stencil_offsets)
+    strides = [0, 0, 0, 0]
+    strides[axis + 2] = 1
+    return _sf.static_sweep(f_data, axis, _sf.block_shifts(blocks, f_data.device), W, strides)
+
+
+def advect_v_static(f_data, E, dt, dv, S, stencil_offsets, axis):
+    """Drop-in for advect_v_static (P2:868-913): Dirichlet kick along axis 2
+    (Ex) or 3 (Ey), exact while |E dt/dv| < S (taps outside the static window
+    are dropped, as there)."""
+    _sf._check_offsets(stencil_offsets)
+    Ny = f_data.shape[1]
+    return _sf.sweep_reflected(f_data, axis, E.to(torch.float64).contiguous(), (Ny, 1, 0, 0), dt, dv, False,
+                               int(S))
+
+
+def _fields(f, physics_mode, kx, ky, inv_ksq, dvx, dvy, Nx, Ny):
+    if physics_mode:
+        return solve_poisson_real(f, kx, ky, inv_ksq, dvx, dvy)
+    z = torch.zeros((Nx, Ny), dtype=torch.float64, device=f.device)
+    return z, z.clone(), z.clone(), z.clone()
+
+
+def vlasov_step_matrix_diag(f, f0, dt, grid_x, grid_y, grid_vx, grid_vy, grid_kx, grid_ky, grid_inv_ksq,
+                            grid_x_indices, grid_y_indices, grid_vx_indices, grid_vy_indices, grid_stencil_offsets,
+                            grid_dx, grid_dy, grid_dvx, grid_dvy, grid_Nx, grid_Ny, physics_mode):
+    """Drop-in for vlasov_step_matrix_diag (P2:1001-1040), the solver-1 step."""
+    so = grid_stencil_offsets
+    f = advect_1d_matrix_diag(f, grid_vx[None, :, None], 0, dt / 2.0, grid_dx, stencil_offsets=so, periodic=True)
+    f = advect_1d_matrix_diag(f, grid_vy[None, None, :], 1, dt / 2.0, grid_dy, stencil_offsets=so, periodic=True)
+    Ex, Ey, rho, phi = _fields(f, physics_mode, grid_kx, grid_ky, grid_inv_ksq, grid_dvx, grid_dvy, grid_Nx,
+                               grid_Ny)
+    f = advect_1d_matrix_diag(f, Ex[:, :, None], 2, dt, grid_dvx, stencil_offsets=so, periodic=False)
+    f = advect_1d_matrix_diag(f, Ey[:, :, None], 3, dt, grid_dvy, stencil_offsets=so, periodic=False)
+    f = advect_1d_matrix_diag(f, grid_vy[None, None, :], 1, dt / 2.0, grid_dy, stencil_offsets=so, periodic=True)
+    f = advect_1d_matrix_diag(f, grid_vx[None, :, None], 0, dt / 2.0, grid_dx, stencil_offsets=so, periodic=True)
+    return f, rho, phi
+
+
+def vlasov_step_gather_free_impl(f, dt, W_x, W_y, blocks_x, blocks_y, stencil_offsets, grid_kx, grid_ky,
+                                 grid_inv_ksq, grid_dvx, grid_dvy, grid_Nx, grid_Ny, physics_mode, S):
+    """Drop-in for vlasov_step_gather_free_impl (P2:1043-1083), the solver-2 core."""
+    f = advect_x_static(f, W_x, blocks_x, stencil_offsets, axis=0)
+    f = advect_x_static(f, W_y, blocks_y, stencil_offsets, axis=1)
+    Ex, Ey, rho, phi = _fields(f, physics_mode, grid_kx, grid_ky, grid_inv_ksq, grid_dvx, grid_dvy, grid_Nx,
+                               grid_Ny)
+    f = advect_v_static(f, Ex, dt, grid_dvx, S, stencil_offsets, axis=2)
+    f = advect_v_static(f, Ey, dt, grid_dvy, S, stencil_offsets, axis=3)
+    f = advect_x_static(f, W_y, blocks_y, stencil_offsets, axis=1)
+    f = advect_x_static(f, W_x, blocks_x, stencil_offsets, axis=0)
+    return f, rho, phi
+
+
+def vlasov_step_gather_free(f, f0, dt, grid_x, grid_y, grid_vx, grid_vy, grid_kx, grid_ky, grid_inv_ksq,
+                            grid_x_indices, grid_y_indices, grid_vx_indices, grid_vy_indices, grid_stencil_offsets,
+                            grid_dx, grid_dy, grid_dvx, grid_dvy, grid_Nx, grid_Ny, physics_mode, S=1):
+    """Drop-in for vlasov_step_gather_free (P2:1086-1132)."""
+    blocks_x, W_x = precompute_x_drift(grid_vx, dt / 2.0, grid_dx, f.dtype)
+    blocks_y, W_y = precompute_x_drift(grid_vy, dt / 2.0, grid_dy, f.dtype)
+    return vlasov_step_gather_free_impl(f, dt, W_x, W_y, blocks_x, blocks_y,
+                                        static_stencil_offsets(grid_stencil_offsets), grid_kx, grid_ky,
+                                        grid_inv_ksq, grid_dvx, grid_dvy, grid_Nx, grid_Ny, physics_mode, S)
+
+
 def moments(f, grid) -> torch.Tensor:
     """(mass, kinetic energy, L2 norm squared) of f [Nx, Ny, Nvx, Nvy]:
     sum f, 1/2 sum f (vx^2 + vy^2), sum f^2, each times dx dy dvx dvy
@@ -316,10 +428,28 @@ class Vlasov:
         self.out_dir = out_dir
         self.physics_mode = physics_mode
         self.epsilon = epsilon
+        solver_fns = {0: vlasov_step_impl, 1: vlasov_step_matrix_diag, 2: vlasov_step_gather_free}
         solver_names = {0: "vlasov_step_impl", 1: "vlasov_step_matrix_diag", 2: "vlasov_step_gather_free"}
         self.solver = solver
         self.solver_type = solver_names[solver]
-        self._step_fn = vlasov_step_impl
+        self._step_fn = solver_fns[solver]
+        if solver != 0 and self.f_dtype != torch.float64:
+            raise ValueError("solvers 1 and 2 run float64 (the FP32-storage variant uses solver 0)")
+        if solver == 2:
+            # static drift data hoisted out of the loop, as in P2:1277-1303
+            g = self.grid
+            bx, Wx = precompute_x_drift(g.vx, self.dt / 2.0, g.dx, self.dtype)
+            by, Wy = precompute_x_drift(g.vy, self.dt / 2.0, g.dy, self.dtype)
+            offsets = static_stencil_offsets(g.stencil_offsets)
+
+            def _step_gather_free(f, f0, dt, grid_x, grid_y, grid_vx, grid_vy, grid_kx, grid_ky, grid_inv_ksq,
+                                  grid_x_indices, grid_y_indices, grid_vx_indices, grid_vy_indices,
+                                  grid_stencil_offsets, grid_dx, grid_dy, grid_dvx, grid_dvy, grid_Nx, grid_Ny,
+                                  physics_mode):
+                return vlasov_step_gather_free_impl(f, dt, Wx, Wy, bx, by, offsets, grid_kx, grid_ky, grid_inv_ksq,
+                                                    grid_dvx, grid_dvy, grid_Nx, grid_Ny, physics_mode, 1)
+
+            self._step_fn = _step_gather_free
         self.energy_history = np.zeros(0)
         self.moment_history: list = []  # (step, t, mass, kinetic, L2) at every snapshot
         pathlib.Path(self.out_dir).mkdir(parents=True, exist_ok=True)
@@ -396,6 +526,8 @@ class Vlasov:
         f = state.variables.fn
         energy = torch.zeros(max(self.nbiter, 1), dtype=torch.float64, device=self.device)
         synchronize(self.device)
+        if self.solver != 0:
+            return self._run_steps(state, energy)
         runner = Run2D(g, f, self.dt, self.physics_mode)
         done = 0
         while done < self.nbiter:
@@ -417,6 +549,39 @@ class Vlasov:
         self.energy_history = energy[:self.nbiter].cpu().numpy()
         return SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=state.variables.f0),
                                t=state.t, step=state.step)
+
+
+def _vlasov_run_steps(self, state, energy):
+    """Step-by-step loop of P2:1392-1435 through ``self._step_fn`` (solvers 1, 2)."""
+    g = self.grid
+    rho, phi = state.field.rho, state.field.phi
+    f = state.variables.fn
+    diag_counter = 0
+    for i in range(self.nbiter):
+        f, rho, phi = self._step_fn(f, state.variables.f0, self.dt, g.x, g.y, g.vx, g.vy, g.kx, g.ky, g.inv_ksq,
+                                    g.x_indices, g.y_indices, g.vx_indices, g.vy_indices, g.stencil_offsets, g.dx,
+                                    g.dy, g.dvx, g.dvy, g.Nx, g.Ny, self.physics_mode)
+        if self.physics_mode:
+            # mid-step field from the step's own charge density (rho returned = delta_rho)
+            Ex, Ey, _, _ = compute_poisson_real(rho + 1.0, g.kx, g.ky, g.inv_ksq)
+            check(lib().vp_field_energy_f64(Ex.data_ptr(), Ey.data_ptr(), g.Nx * g.Ny, g.dx, g.dy,
+                                            energy.data_ptr() + 8 * i, stream_handle(self.device)),
+                  "vp_field_energy_f64")
+        state.t += self.dt
+        state.step += 1
+        diag_counter += 1
+        if diag_counter >= self.diag_steps:
+            synchronize(self.device)
+            self.save_diag(SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=state.variables.f0),
+                                           t=state.t, step=state.step))
+            diag_counter = 0
+    synchronize(self.device)
+    self.energy_history = energy[:self.nbiter].cpu().numpy()
+    return SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=state.variables.f0), t=state.t,
+                           step=state.step)
+
+
+Vlasov._run_steps = _vlasov_run_steps
 
 
 def main(argv=None) -> str:

@@ -303,6 +303,121 @@ def vlasov_step_2d(f, dt, g: Grid2D, physics_mode=True, grid_arrays=None):
 
 
 # --------------------------------------------------------------------------
+# Solver-1/2 forms (reflected stencil)
+# --------------------------------------------------------------------------
+
+def advect_rows_reflected(f_batch, velocities, dt, dx, periodic=True, window=-1):
+    """Shift form of the sweep on f_batch [B, N] (one velocity per row or per
+    point): d = (v dt)/dx, s = floor(d), alpha = d - s, out[i] = sum_k
+    w_k(alpha) f[i - s - o_k], o_k = k - 2, accumulated from 0 in k order.
+    Follows advect_1d_matrix_diag (P1:283-320: _roll_columns/_roll_rows and
+    build_v_diag_weights) and, with window >= 0, advect_v_static
+    (P1:479-501), which keeps only taps m = s + o_k in [-window-2, window+2].
+    """
+    f = np.asarray(f_batch, dtype=np.float64)
+    B, N = f.shape
+    vel = np.broadcast_to(np.asarray(velocities, dtype=np.float64).reshape(B, -1), (B, N))
+    d = (vel * dt) / dx
+    s = np.floor(d)
+    w = lagrange5_weights(d - s)
+    s = s.astype(np.int64)
+    i = np.arange(N)[None, :]
+    out = np.zeros_like(f)
+    rows = np.arange(B)[:, None]
+    for k in range(6):
+        o = k - 2
+        j = i - s - o
+        if periodic:
+            vals = f[rows, j % N]
+        else:
+            valid = (j >= 0) & (j < N)
+            if window >= 0:
+                m = s + o
+                valid &= (m >= -(window + 2)) & (m <= window + 2)
+            vals = np.where(valid, f[rows, np.clip(j, 0, N - 1)], 0.0)
+        out = out + vals * w[k]
+    return out
+
+
+def advect_axis_reflected(f, velocities, axis, dt, dx, periodic=True, window=-1):
+    """``advect_rows_reflected`` along any axis (velocities broadcast against
+    f with the swept axis removed, or of f's full shape)."""
+    f = np.asarray(f, dtype=np.float64)
+    v = np.asarray(velocities, dtype=np.float64)
+    fm = np.moveaxis(f, axis, -1)
+    if v.ndim == f.ndim and v.shape == f.shape:
+        vm = np.moveaxis(v, axis, -1)
+    else:
+        vm = np.broadcast_to(np.moveaxis(np.broadcast_to(v, f.shape[:axis] + (1,) + f.shape[axis + 1:]), axis, -1),
+                             fm.shape)
+    out = advect_rows_reflected(fm.reshape(-1, fm.shape[-1]), vm.reshape(-1, fm.shape[-1]), dt, dx, periodic, window)
+    return np.moveaxis(out.reshape(fm.shape), -1, axis)
+
+
+def precompute_x_drift(v, dt, dx):
+    """(blocks, W) of P1:357-387: s = floor((v dt)/dx) in contiguous
+    (start, stop, shift) blocks and W = w_k(alpha) [6, Nv]."""
+    d = (np.asarray(v, dtype=np.float64) * dt) / dx
+    s = np.floor(d)
+    alpha = d - s
+    s = s.astype(np.int64)
+    edges = np.flatnonzero(np.diff(s)) + 1
+    starts = np.concatenate(([0], edges))
+    stops = np.concatenate((edges, [len(s)]))
+    blocks = [(int(a), int(b), int(s[a])) for a, b in zip(starts, stops)]
+    return blocks, lagrange5_weights(alpha)
+
+
+def advect_x_static(f, W, blocks, axis=0):
+    """advect_x_static (P1:408-448; P2:756-819 with axis): periodic sweep
+    along `axis` with the velocity index on axis + (1 in 1D, 2 in 2D)."""
+    f = np.asarray(f, dtype=np.float64)
+    vel_axis = 1 if f.ndim == 2 else axis + 2
+    shifts = np.array([s for a, b, s in blocks for _ in range(b - a)], dtype=np.int64)
+    n = f.shape[axis]
+    shape = [1] * f.ndim
+    shape[vel_axis] = len(shifts)
+    sh = shifts.reshape(shape)
+    ishape = [1] * f.ndim
+    ishape[axis] = n
+    rows = np.arange(n).reshape(ishape) - sh
+    out = np.zeros_like(f)
+    for k in range(6):
+        idx = np.broadcast_to(np.remainder(rows - (k - 2), n), f.shape)
+        out = out + np.take_along_axis(f, idx, axis=axis) * np.asarray(W[k]).reshape(shape)
+    return out
+
+
+def build_v_tap_weights(E, dt, dv, S):
+    """[2S+5, ...] tap weights of P1:450-477 on m = -S-2 .. S+2."""
+    d = (np.asarray(E, dtype=np.float64) * dt) / dv
+    s = np.floor(d)
+    W6 = lagrange5_weights(d - s)
+    s = s.astype(np.int64)
+    rows = []
+    for m in range(-(S + 2), S + 3):
+        acc = np.zeros_like(d)
+        for k in range(6):
+            acc = acc + (s == m - (k - 2)).astype(np.float64) * W6[k]
+        rows.append(acc)
+    return np.stack(rows)
+
+
+def vlasov_step_1d_reflected(f, dt, v, inv_kx, inv_kx_sq, dx, dv, physics_mode=True, window=-1):
+    """vlasov_step_matrix_diag (P1:625-665, window < 0) and the gather-free
+    step (P1:668-755, window = S) on f [Nx, Nv]: (f, rho, phi, E)."""
+    f = np.asarray(f, dtype=np.float64)
+    f = advect_rows_reflected(f.T, v, dt / 2.0, dx, periodic=True).T
+    if physics_mode:
+        E, rho, phi = solve_poisson_real_1d(f, inv_kx, inv_kx_sq, dv)
+    else:
+        E = rho = phi = np.zeros(f.shape[0])
+    f = advect_rows_reflected(f, E, dt, dv, periodic=False, window=window)
+    f = advect_rows_reflected(f.T, v, dt / 2.0, dx, periodic=True).T
+    return np.ascontiguousarray(f), rho, phi, E
+
+
+# --------------------------------------------------------------------------
 # Diagnostics (no reference function; defined by this build)
 # --------------------------------------------------------------------------
 
