@@ -70,7 +70,7 @@ int vp_advect_f32(const float* f_in, float* f_out, const int64_t shape[4], int a
                   double dt, double dx, int periodic, vp_stream_t stream);
 
 /* Reflected (shift) form of the sweep, used by the reference's solver 1
- * (advect_1d_matrix_diag, P1:283-320, P2:560-641) and solver 2
+ * (advect_1d_matrix_diag, P1:258-337, P2:557-641) and solver 2
  * (advect_v_static, P1:479-501, P2:868-913): d = (vel*dt)/dx,
  * s = floor(d), alpha = d - s, out[i] = sum_k w_k(alpha) in[i - s - o_k]
  * with o_k = k - 2 (taps in that order, wrapped or zero outside).  Agrees

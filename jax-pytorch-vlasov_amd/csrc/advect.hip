@@ -69,7 +69,7 @@ __device__ __forceinline__ T interp_from_lds(const T* tile, int64_t il, double a
 //   kDeparture: the reference sweep (advect_1d_vectorized, vlasov1D1V.py:212-256):
 //     dep = i - (v dt)/dx, il = floor(dep), alpha = dep - il, taps il - 2 .. il + 3.
 //   kReflected: the roll/shift forms of solvers 1 and 2 (advect_1d_matrix_diag,
-//     vlasov1D1V.py:283-320; advect_v_static, vlasov1D1V.py:479-501):
+//     vlasov1D1V.py:258-337; advect_v_static, vlasov1D1V.py:479-501):
 //     d = (v dt)/dx, s = floor(d), alpha = d - s, tap k (offset o = k - 2) reads
 //     i - s - o with weight w_k(alpha); `window` >= 0 keeps only taps with
 //     |s + o| <= window + 2 (the static window of solver 2), < 0 keeps all.

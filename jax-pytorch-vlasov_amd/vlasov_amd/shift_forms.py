@@ -118,7 +118,7 @@ def check_v_window(E, dt, dv, S):
 
 def build_diag_weights(stencil_offsets, shifts, alpha, n_points):
     """[6, B, N] banded weights of the solver-1 Dirichlet sweep
-    (build_v_diag_weights, vlasov1D1V.py:280-300)."""
+    (build_v_diag_weights, vlasov1D1V.py:279-296)."""
     W = lagrange5_weights(alpha)
     v = torch.arange(n_points, device=alpha.device)
     offs = stencil_offsets if isinstance(stencil_offsets, torch.Tensor) else torch.tensor(

@@ -298,7 +298,7 @@ check_v_window = _sf.check_v_window
 
 
 def build_v_diag_weights(stencil_offsets, shifts, alpha, n_points):
-    """[6, Nx, Nv] banded weights of the solver-1 V sweep (P1:280-300)."""
+    """[6, Nx, Nv] banded weights of the solver-1 V sweep (P1:279-296)."""
     return _sf.build_diag_weights(stencil_offsets, shifts, alpha, n_points)
 
 

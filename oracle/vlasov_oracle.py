@@ -31,7 +31,17 @@ How each function is pinned (tests/test_oracle_*.py):
 * ``vlasov_step_1d`` / ``vlasov_step_2d``: dt=0 is the identity, mass
   conservation, Strang composition with E=0, the Landau damping rate against
   the root of the plasma dispersion relation.
-* ``diagnostics_*``: closed forms on separable inputs.
+* ``field_energy_*`` / ``moments_*``: closed forms on separable inputs.
+* ``advect_rows_reflected`` / ``advect_axis_reflected`` (solver 1): integer
+  shifts are exact rolls or zero-filled shifts, agreement with the departure
+  form to roundoff (tests/test_oracle_reflected.py).
+* ``precompute_x_drift`` / ``advect_x_static`` / ``build_v_tap_weights``
+  (solver 2): the static drift equals the reflected sweep, tap weights sum to
+  one, the window is exact inside |E dt/dv| < S and drops outer taps beyond.
+* ``vlasov_step_1d_reflected``: agreement with ``vlasov_step_1d`` to 1e-10
+  (the reference's own tolerance, T2/test_advection.py:526).
+* Committed fixtures (tests/golden/, written from this module) are re-derived
+  by tests/test_golden.py, which also checks their Landau damping rate.
 """
 
 from __future__ import annotations
@@ -310,7 +320,7 @@ def advect_rows_reflected(f_batch, velocities, dt, dx, periodic=True, window=-1)
     """Shift form of the sweep on f_batch [B, N] (one velocity per row or per
     point): d = (v dt)/dx, s = floor(d), alpha = d - s, out[i] = sum_k
     w_k(alpha) f[i - s - o_k], o_k = k - 2, accumulated from 0 in k order.
-    Follows advect_1d_matrix_diag (P1:283-320: _roll_columns/_roll_rows and
+    Follows advect_1d_matrix_diag (P1:258-337: _roll_columns/_roll_rows and
     build_v_diag_weights) and, with window >= 0, advect_v_static
     (P1:479-501), which keeps only taps m = s + o_k in [-window-2, window+2].
     """
