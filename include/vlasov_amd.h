@@ -306,9 +306,15 @@ enum {
     VP_KNOB_X_SPLIT = 0,       /* y-range splits of the X drift grid            */
     VP_KNOB_Y_SPLIT = 1,       /* vx-range splits of the Y drift grid           */
     VP_KNOB_KICK_PREFETCH = 2, /* row-streaming kick: 2 = 16 rows ahead, else 8;
+                                  row-split kick: 1 = 4 rows, 3 = 8 rows
+                                  (else 4 for double, 8 for float);
                                   in-place plane kick: != 0 prefetches the next
                                   plane (default 2)                            */
-    VP_KNOB_KICK_KERNEL = 3,   /* 1: whole-plane LDS kick even out of place      */
+    VP_KNOB_KICK_KERNEL = 3,   /* 1: whole-plane LDS kick even out of place;
+                                  at 128 vy columns: 2 = two-wave
+                                  row-streaming kick, 3/4/5/6 = row-split
+                                  kick with 2/4/8/16 waves per plane
+                                  (0: 8 waves per plane)                       */
     VP_KNOB_DRIFT_DMA = 4,     /* drifts at 64/128 rows: 1 = register prefetch,
                                   2 or 3 = direct-to-LDS ring of that many
                                   tiles (0 = 2)                                */

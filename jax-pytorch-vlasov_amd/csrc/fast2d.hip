@@ -638,6 +638,202 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
     }
 }
 
+// Row-split Vx+Vy kick of a (vx, vy) plane with 128 vy columns (out of
+// place).  Same arithmetic as kick_stream_kernel, other work split: WPP
+// waves per plane, wave w owns the RPW = NVX / WPP consecutive rows
+// [w RPW, (w + 1) RPW), and lane l owns the column pair (2l, 2l+1), so every
+// row moves as one 2-element vector load and store per lane (16 bytes for
+// double).  Streams of 32 rows keep HBM busier than one stream per 128-row
+// plane.  Even waves walk their rows downwards, odd waves upwards, so two
+// neighbours reach their shared boundary (and the up-to-5 input rows both
+// read there) at the same time and the second read finds the rows on chip.
+// The Vx window slides with the walk; the six taps are always summed from
+// the lowest row up, as in the reference.  The Vy taps of the just-computed
+// row come from a zero-padded per-wave LDS row: taps outside [0, NVY) read 0
+// there instead of being clamped and multiplied by a 0/1 mask (the same
+// values: the mask product is +-0 and adding it changes no nonzero sum).
+// When every lane's two columns have adjacent tap windows (the normal case)
+// the pair shares one 7-value read.  A wave's LDS row is written and read by
+// that wave only: the write is waited for (lgkmcnt) before the reads, no
+// workgroup barrier.
+template <typename S, int NVX, int WPP, int PF, bool NT>
+__global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict__ fin, S* __restrict__ fout,
+                                                             const double* __restrict__ Ex,
+                                                             const double* __restrict__ Ey, double dt, double dvx,
+                                                             double dvy) {
+    constexpr int NVY = 128;
+    constexpr int PAD = 16;  // >= 10 on the left, >= 12 on the right (taps of l in [-8, NVY + 8])
+    constexpr int RPW = NVX / WPP;
+    static_assert(NVX % WPP == 0 && RPW % PF == 0, "rows per wave must be a multiple of PF");
+    typedef S V2 __attribute__((ext_vector_type(2)));
+    __shared__ S wx[NVX * 6];
+    __shared__ int ilx[NVX];
+    __shared__ int irregular[1];
+    __shared__ S rowbuf[WPP][2][NVY + 2 * PAD];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int64_t p = blockIdx.x;  // the host launches exactly one workgroup per plane
+
+    if (t == 0) irregular[0] = 0;
+    const double ex = Ex[p];
+    for (int r = t; r < NVX; r += 64 * WPP) {
+        const Departure d = departure(r, ex, dt, dvx);
+        double w[6];
+        lagrange5_weights<double>(d.alpha, w);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wx[r * 6 + k] = static_cast<S>(w[k]);
+        ilx[r] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
+    }
+    // zero pads of the wave's two row buffers: 2 x 2 x PAD = 64 entries, one per lane
+    {
+        const int b = lane >> 5, i = lane & 31;
+        rowbuf[wv][b][i < PAD ? i : NVY + i] = S(0);
+    }
+    S wy[2][6];
+    int ty[2];  // LDS index of tap 0 of each column
+    {
+        const double ey = Ey[p];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const Departure d = departure(2 * lane + j, ey, dt, dvy);
+            double w[6];
+            lagrange5_weights<double>(d.alpha, w);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wy[j][k] = static_cast<S>(w[k]);
+            const int l = static_cast<int>(d.il < -8 ? -8 : (d.il > NVY + 8 ? NVY + 8 : d.il));
+            ty[j] = PAD + l - 2;
+        }
+    }
+    const bool pair = __ballot(ty[1] != ty[0] + 1) == 0;  // wave-uniform
+    __syncthreads();
+    const int s0 = ilx[0];
+    for (int r = t; r < NVX; r += 64 * WPP)
+        if (ilx[r] - r != s0) irregular[0] = 1;
+    __syncthreads();
+    const bool regular = irregular[0] == 0;  // uniform over the workgroup
+
+    const S* src = fin + p * NVX * NVY + 2 * lane;
+    S* dst = fout + p * NVX * NVY + 2 * lane;
+    auto ld = [&](int row) -> V2 {
+        const V2* a = reinterpret_cast<const V2*>(src + row * NVY);
+        return NT ? __builtin_nontemporal_load(a) : *a;
+    };
+    auto clampr = [](int j) { return j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j); };
+    auto maskr = [](int j) { return (j >= 0 && j < NVX) ? S(1) : S(0); };
+    // Vy kick of row r from the Vx results ox of the lane's two columns.
+    auto vy_row = [&](int r, S ox0, S ox1) {
+        S* rb = rowbuf[wv][r & 1];
+        V2 o;
+        o.x = ox0;
+        o.y = ox1;
+        *reinterpret_cast<V2*>(&rb[PAD + 2 * lane]) = o;
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        S a[6], b[6];
+        if (pair) {
+            S v[7];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) v[i] = rb[ty[0] + i];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                a[k] = v[k];
+                b[k] = v[k + 1];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                a[k] = rb[ty[0] + k];
+                b[k] = rb[ty[1] + k];
+            }
+        }
+        V2 res;
+        res.x = tap_sum<S>(wy[0], a);
+        res.y = tap_sum<S>(wy[1], b);
+        V2* out = reinterpret_cast<V2*>(dst + r * NVY);
+        if (NT)
+            __builtin_nontemporal_store(res, out);
+        else
+            *out = res;
+    };
+    auto vx_pair = [&](int r, const V2 v[6], S& ox0, S& ox1) {
+        S w[6], a[6], b[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            w[k] = wx[r * 6 + k];
+            a[k] = v[k].x;
+            b[k] = v[k].y;
+        }
+        ox0 = tap_sum<S>(w, a);
+        ox1 = tap_sum<S>(w, b);
+    };
+
+    const int r_lo = wv * RPW;
+    const bool down = WPP > 1 && (wv & 1) == 0;
+    if (regular && !down) {
+        // rows r_lo .. r_lo + RPW - 1 upwards; window = taps 0..4 of row r
+        V2 win[5], q[PF];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) win[k] = ld(clampr(r_lo + s0 - 2 + k)) * maskr(r_lo + s0 - 2 + k);
+#pragma unroll
+        for (int u = 0; u < PF; ++u) q[u] = ld(clampr(r_lo + s0 + 3 + u));
+        for (int i0 = 0; i0 < RPW; i0 += PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int r = r_lo + i0 + u;
+                const int jn = r + s0 + 3;  // row held by q[u]: tap 5 of row r
+                const V2 v5 = q[u] * maskr(jn);
+                const V2 v[6] = {win[0], win[1], win[2], win[3], win[4], v5};
+                S ox0, ox1;
+                vx_pair(r, v, ox0, ox1);
+                win[0] = win[1];
+                win[1] = win[2];
+                win[2] = win[3];
+                win[3] = win[4];
+                win[4] = v5;
+                if (i0 + u + PF < RPW) q[u] = ld(clampr(jn + PF));
+                vy_row(r, ox0, ox1);
+            }
+        }
+    } else if (regular) {
+        // rows r_lo + RPW - 1 .. r_lo downwards; window = taps 1..5 of row r
+        const int r_hi = r_lo + RPW - 1;
+        V2 win[5], q[PF];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) win[k] = ld(clampr(r_hi + s0 - 1 + k)) * maskr(r_hi + s0 - 1 + k);
+#pragma unroll
+        for (int u = 0; u < PF; ++u) q[u] = ld(clampr(r_hi + s0 - 2 - u));
+        for (int i0 = 0; i0 < RPW; i0 += PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int r = r_hi - i0 - u;
+                const int jn = r + s0 - 2;  // row held by q[u]: tap 0 of row r
+                const V2 v0 = q[u] * maskr(jn);
+                const V2 v[6] = {v0, win[0], win[1], win[2], win[3], win[4]};
+                S ox0, ox1;
+                vx_pair(r, v, ox0, ox1);
+                win[4] = win[3];
+                win[3] = win[2];
+                win[2] = win[1];
+                win[1] = win[0];
+                win[0] = v0;
+                if (i0 + u + PF < RPW) q[u] = ld(clampr(jn - PF));
+                vy_row(r, ox0, ox1);
+            }
+        }
+    } else {
+        for (int i = 0; i < RPW; ++i) {
+            const int r = r_lo + i;
+            const int l = ilx[r];
+            V2 v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = ld(clampr(l + k - 2)) * maskr(l + k - 2);
+            S ox0, ox1;
+            vx_pair(r, v, ox0, ox1);
+            vy_row(r, ox0, ox1);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -894,11 +1090,58 @@ static int launch_kick_stream(const S* fin, S* fout, int nplanes, const double* 
     return VP_OK;
 }
 
+// Row-split kick, 128 vy columns, WPP waves per plane, pf8: 8 rows
+// prefetched (else 4), at most the rows per wave.
+template <typename S, int NVX, int WPP>
+static int launch_kick_rows(bool pf8, const S* fin, S* fout, int nplanes, const double* Ex, const double* Ey,
+                            double dt, double dvx, double dvy, hipStream_t s) {
+    constexpr int RPW = NVX / WPP;
+    constexpr int PF8 = RPW < 8 ? RPW : 8;
+    constexpr int PF4 = RPW < 4 ? RPW : 4;
+    const bool nt = (nt_mask() & 4) != 0;
+#define VP_K(PF, NT)                                                                                               \
+    hipLaunchKernelGGL((kick_rows_kernel<S, NVX, WPP, PF, NT>), dim3(nplanes), dim3(64 * WPP), 0, s, fin, fout, Ex, \
+                       Ey, dt, dvx, dvy)
+    if (pf8) {
+        if (nt) VP_K(PF8, true); else VP_K(PF8, false);
+    } else {
+        if (nt) VP_K(PF4, true); else VP_K(PF4, false);
+    }
+#undef VP_K
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+// Kick kernel choice at 128 vy columns: the kick-kernel knob 3, 4, 5, 6 picks
+// the row-split kernel with 2, 4, 8, 16 waves per plane, 2 the two-wave
+// row-streaming kernel; the kick-prefetch knob 1 or 3 picks 4 or 8 rows
+// prefetched.  Automatic (knobs 0 and 2): 8 waves per plane, 4 rows
+// prefetched for double and 8 for float (measured fastest at 128^4).
+template <typename S, int NVX>
+static int kick_rows_dispatch(int kk, int pfk, const S* fin, S* fout, int nplanes, const double* Ex,
+                              const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
+    if (kk == 0) kk = 5;
+    const bool pf8 = pfk == 3 || (pfk != 1 && sizeof(S) == 4);
+    if (kk == 3) return launch_kick_rows<S, NVX, 2>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+    if (kk == 4) return launch_kick_rows<S, NVX, 4>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+    if (kk == 5) return launch_kick_rows<S, NVX, 8>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+    if (kk == 6 && NVX >= 64) return launch_kick_rows<S, NVX, 16>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+    return VP_ERR_UNSUPPORTED;  // row-streaming kernel
+}
+
 template <typename S>
 static int kick_stream(const S* fin, S* fout, const int64_t shape[4], const double* Ex, const double* Ey, double dt,
                        double dvx, double dvy, hipStream_t s) {
     const int64_t nvx = shape[2], nvy = shape[3];
     const int nplanes = static_cast<int>(shape[0] * shape[1]);
+    if (nvy == 128) {
+        const int kk = knob(VP_KNOB_KICK_KERNEL), pfk = knob(VP_KNOB_KICK_PREFETCH);
+        int rc = VP_ERR_UNSUPPORTED;
+        if (nvx == 128) rc = kick_rows_dispatch<S, 128>(kk, pfk, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+        if (nvx == 64) rc = kick_rows_dispatch<S, 64>(kk, pfk, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+        if (nvx == 32) rc = kick_rows_dispatch<S, 32>(kk, pfk, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+        if (rc != VP_ERR_UNSUPPORTED) return rc;
+    }
 #define VP_KS(A, B) \
     if (nvx == A && nvy == B) return launch_kick_stream<S, A, B>(fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
     VP_KS(128, 128)

@@ -132,3 +132,41 @@ def test_fp32_rejects_in_place_kick_and_small_shapes(device):
     rc = lib.vp_fused_kick_f32(f.data_ptr(), f.data_ptr(), L.i64x4(shape), E.data_ptr(), E.data_ptr(), 0.1, 0.1,
                                0.1, torch.cuda.current_stream().cuda_stream)
     assert rc == -4  # VP_ERR_UNSUPPORTED: FP32 kick runs out of place only
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 128, 128), (8, 4, 64, 128), (4, 8, 128, 64)])
+def test_fp32_kick_kernels_agree(device, shape):
+    """The float32 row-split kick (128 vy columns, 2/4/8/16 waves per plane,
+    4/8 prefetched rows) and
+    the two-wave row-streaming kick give the same bits, with multi-cell and
+    out-of-domain shifts in the field."""
+    v2, runner, L = _mods()
+    lib = L.lib()
+    nx, ny, nvx, nvy = shape
+    rng = np.random.default_rng(5)
+    f = torch.from_numpy(rng.random(shape)).float().to(device)
+    dt, dvx, dvy = 0.1, 12.0 / (nvx - 1), 12.0 / (nvy - 1)
+    choices = np.array([0.0, 1e-18, 0.4, -0.7, 2.0, 5.3, -5.3, 80.0, -80.0])
+    Ex = torch.from_numpy(rng.choice(choices, size=(nx, ny)) * dvx / dt).to(device)
+    Ey = torch.from_numpy(rng.choice(choices, size=(nx, ny)) * dvy / dt).to(device)
+    sh = L.i64x4(shape)
+    s = torch.cuda.current_stream().cuda_stream
+    outs = []
+    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("kick_kernel", "kick_prefetch")}
+    try:
+        for kk, pf in ((0, 2), (2, 1), (3, 3), (4, 1), (4, 3), (5, 1), (6, 1), (6, 3)):
+            lib.vp_tune_set(L.KNOBS["kick_kernel"], kk)
+            lib.vp_tune_set(L.KNOBS["kick_prefetch"], pf)
+            o = torch.empty_like(f)
+            L.check(lib.vp_fused_kick_f32(f.data_ptr(), o.data_ptr(), sh, Ex.data_ptr(), Ey.data_ptr(), dt, dvx, dvy,
+                                          s), "vp_fused_kick_f32")
+            outs.append(o)
+    finally:
+        for k, v in old.items():
+            lib.vp_tune_set(L.KNOBS[k], v)
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    # and against the generic float32 sweeps (same arithmetic)
+    a = v2.advect_1d(f, Ex[:, :, None], 2, dt, dvx, periodic=False)
+    a = v2.advect_1d(a, Ey[:, :, None], 3, dt, dvy, periodic=False)
+    assert torch.equal(outs[0], a)

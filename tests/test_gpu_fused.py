@@ -153,18 +153,20 @@ def test_vlasov2d_driver_fused_landau(device, tmp_path):
     assert st.step == 20 and (tmp_path / "field_000010.npz").exists() and (tmp_path / "field_000020.npz").exists()
 
 
-def test_kick_stream_and_plane_kernels_agree_on_edge_fields(device):
-    """Out-of-place (row-streaming) and in-place (whole-plane LDS) kicks and the
-    oracle agree bit for bit on fields that exercise every tap path: E = 0,
-    tiny E (rounding makes floor(r - d) - r non-constant), multi-cell and
-    fully out-of-domain shifts, both signs."""
+@pytest.mark.parametrize("shape", [(8, 4, 64, 128), (4, 4, 128, 128), (4, 8, 32, 128), (8, 8, 128, 64)])
+def test_kick_stream_and_plane_kernels_agree_on_edge_fields(device, shape):
+    """Out-of-place kicks (row-split kernel at 128 vy columns with 2, 4, 8 and
+    16 waves per plane and 4 or 8 prefetched rows, two-wave row-streaming
+    kernel), the in-place whole-plane
+    LDS kick and the oracle agree bit for bit on fields that exercise every
+    tap path: E = 0, tiny E (rounding makes floor(r - d) - r non-constant),
+    multi-cell and fully out-of-domain shifts, both signs."""
     v2, runner, L = _mods()
     lib = L.lib()
-    shape = (8, 4, 64, 128)
     nx, ny, nvx, nvy = shape
     rng = np.random.default_rng(11)
     f = rng.random(shape)
-    dt, dvx, dvy = 0.1, 12.0 / 63, 12.0 / 127
+    dt, dvx, dvy = 0.1, 12.0 / (nvx - 1), 12.0 / (nvy - 1)
     choices = np.array([0.0, 1e-18, -1e-18, 3e-17 * dvx / dt, 5.3 * dvx / dt, -5.3 * dvx / dt,
                         0.4 * dvx / dt, -0.7 * dvx / dt, 80.0 * dvx / dt, -80.0 * dvx / dt, 2.0 * dvx / dt])
     Ex = rng.choice(choices, size=(nx, ny))
@@ -178,6 +180,21 @@ def test_kick_stream_and_plane_kernels_agree_on_edge_fields(device):
     sh = L.i64x4(shape)
     L.check(lib.vp_fused_kick_f64(tf.data_ptr(), out.data_ptr(), sh, tEx.data_ptr(), tEy.data_ptr(), dt, dvx, dvy,
                                   s), "kick out of place")
+    outs = []
+    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("kick_kernel", "kick_prefetch")}
+    try:
+        for kk, pf in ((0, 1), (0, 3), (2, 2), (2, 1), (3, 3), (4, 1), (5, 3), (5, 1), (6, 3), (6, 1)):
+            lib.vp_tune_set(L.KNOBS["kick_kernel"], kk)
+            lib.vp_tune_set(L.KNOBS["kick_prefetch"], pf)
+            o = torch.empty_like(tf)
+            L.check(lib.vp_fused_kick_f64(tf.data_ptr(), o.data_ptr(), sh, tEx.data_ptr(), tEy.data_ptr(), dt, dvx,
+                                          dvy, s), "kick out of place")
+            outs.append(o)
+    finally:
+        for k, v in old.items():
+            lib.vp_tune_set(L.KNOBS[k], v)
+    for o in outs:
+        assert torch.equal(o, out)
     inplace = tf.clone()
     L.check(lib.vp_fused_kick_f64(inplace.data_ptr(), inplace.data_ptr(), sh, tEx.data_ptr(), tEy.data_ptr(), dt,
                                   dvx, dvy, s), "kick in place")

@@ -34,7 +34,7 @@ def test_landau1d_history_and_f(device):
     g = v1.Grid.create(c["Nx"], c["Nv"], c["L"], c["V_max"], device=device)
     _, f0 = mg.landau1d_initial()
     f = torch.from_numpy(f0).to(device)
-    _, _, E0 = v1.solve_poisson_real(f, g.inv_kx, g.inv_kx_sq, g.dv)
+    E0, _, _ = v1.solve_poisson_real(f, g.inv_kx, g.inv_kx_sq, g.dv)
     W = [v1.field_energy(E0, g.dx)]
     for s in range(1, c["steps"] + 1):
         f, _, _, E = v1.step_with_field(f, c["dt"], g.v, g.inv_kx, g.inv_kx_sq, g.dx, g.dv, True)
