@@ -188,7 +188,8 @@ int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_t shape[4],
  * FP32-storage variant of the fused run (no reference counterpart: the
  * reference's float32 mode computes everything in float32).  f is float32
  * in HBM and interpolated in float32 with weights computed in float64 and
- * rounded once; the charge density is summed in float64 from the stored
+ * rounded once (the smallest one balanced so the six sum to 1 within half
+ * its ulp); the charge density is summed in float64 from the stored
  * values, and the Poisson solve, fields and field energy stay float64.
  * Same arguments and step order as vp_run_2d2v_f64.  Shapes: nx, ny, nvy in
  * {64, 128}, nvx in {32, 64, 128} (vp_run_2d2v_f32_workspace returns 0
@@ -213,6 +214,45 @@ int vp_fused_poisson_f32(const int64_t shape[4], double dvx, double dvy, const d
                          vp_stream_t stream);
 int vp_fused_kick_f32(const float* f_in, float* f_out, const int64_t shape[4], const double* Ex,
                       const double* Ey, double dt, double dvx, double dvy, vp_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * FP32 deviation storage (no reference counterpart): f = feq(vx, vy) +
+ * g(x, y, vx, vy) with feq the (x, y) mean of the initial f rounded to
+ * float32 and g in float32.  feq is defined as those float32 values (every
+ * use widens them exactly to float64), so storing it loses nothing.  feq is
+ * constant in x and y, so the drifts leave it unchanged and run on g (the
+ * vp_fused_*drift_f32 kernels); the kick evaluates feq + g in float64 and
+ * stores the new deviation; the charge density is sum(g) dv + sum(feq) dv.
+ * Storage rounding then scales with the perturbation |g| instead of |f|:
+ * the field-energy history stays within ~1e-6 of the float64 one where
+ * plain FP32 storage drifts to ~1e-5.  Shapes: those of vp_run_2d2v_f32
+ * with nvy == 128.
+ *
+ * vp_dev_split_f32: feq[m] = float(mean over the nx*ny planes of f)
+ *   (compensated), g = float(f - feq), rho_shift[0] = ((sum feq) * dvx) *
+ *   dvy - 1 (device scalar); ws of vp_dev_split_workspace(shape) bytes.
+ * vp_dev_join_f32: f = feq + g into f64_out and/or (rounded) f32_out.
+ * vp_run_2d2v_dev_f32: vp_run_2d2v_f32 on g (same arguments and ws layout).
+ * vp_fused_kick_dev_f32 / vp_fused_poisson_dev_f32: the two pieces that
+ *   differ from the vp_fused_*_f32 ones (the drifts and tables are shared).
+ * --------------------------------------------------------------------- */
+size_t vp_dev_split_workspace(const int64_t shape[4]);
+int vp_dev_split_f32(const double* f, const int64_t shape[4], double dvx, double dvy, float* feq,
+                     double* rho_shift, float* g, void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_dev_join_f32(const float* g, const float* feq, const int64_t shape[4], float* f32_out,
+                    double* f64_out, vp_stream_t stream);
+int vp_run_2d2v_dev_f32(float* g, const float* feq, const double* rho_shift, const int64_t shape[4],
+                        int nsteps, double dt, const double* vx, const double* vy, const double* kx,
+                        const double* ky, const double* inv_ksq, double dx, double dy, double dvx,
+                        double dvy, int physics_mode, double* Ex, double* Ey, double* delta_rho,
+                        double* phi, double* energy, void* ws, size_t ws_bytes, vp_stream_t stream);
+int vp_fused_kick_dev_f32(const float* g_in, float* g_out, const int64_t shape[4], const float* feq,
+                          const double* Ex, const double* Ey, double dt, double dvx, double dvy,
+                          vp_stream_t stream);
+int vp_fused_poisson_dev_f32(const int64_t shape[4], double dvx, double dvy, const double* rho_shift,
+                             const double* kx, const double* ky, const double* inv_ksq, double* Ex,
+                             double* Ey, double* delta_rho, double* phi, void* ws, size_t ws_bytes,
+                             vp_stream_t stream);
 
 /* ---------------------------------------------------------------------
  * Diagnostics (no reference counterpart; the reference only writes

@@ -41,8 +41,7 @@ template <typename T>
 __device__ __forceinline__ void weights_as(double alpha, T w[6]) {
     double wd[6];
     lagrange5_weights<double>(alpha, wd);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = static_cast<T>(wd[k]);
+    round_weights<T>(wd, w);
 }
 
 template <typename T, bool PERIODIC>

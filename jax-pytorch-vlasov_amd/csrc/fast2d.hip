@@ -21,6 +21,8 @@
 //
 // Tiles and planes are disjoint between workgroups and each is read whole
 // before it is written, so all three kernels may run in place.
+#include <type_traits>
+
 #include "fast2d.hpp"
 
 namespace vp {
@@ -274,8 +276,10 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         const int r = g + KG * i;
+        double wd[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) w[i][k] = static_cast<S>(W[(static_cast<int64_t>(r) * 6 + k) * nvel + j]);
+        for (int k = 0; k < 6; ++k) wd[k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
+        round_weights<S>(wd, w[i]);
         const int il = IL[static_cast<int64_t>(r) * nvel + j];
         A[i] = static_cast<uint32_t>(((il - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(S)));
     }
@@ -553,8 +557,10 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
         const Departure d = departure(r, ex, dt, dvx);
         double w[6];
         lagrange5_weights<double>(d.alpha, w);
+        S ws[6];
+        round_weights<S>(w, ws);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) wx[pg][r * 6 + k] = static_cast<S>(w[k]);
+        for (int k = 0; k < 6; ++k) wx[pg][r * 6 + k] = ws[k];
         ilx[pg][r] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
     }
     S wy[6], my[6];
@@ -563,8 +569,7 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
         const Departure d = departure(c, Ey[p], dt, dvy);
         double wyd[6];
         lagrange5_weights<double>(d.alpha, wyd);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) wy[k] = static_cast<S>(wyd[k]);
+        round_weights<S>(wyd, wy);
         const int l = static_cast<int>(d.il < -8 ? -8 : (d.il > NVY + 8 ? NVY + 8 : d.il));
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -643,8 +648,9 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
 // waves per plane, wave w owns the RPW = NVX / WPP consecutive rows
 // [w RPW, (w + 1) RPW), and lane l owns the column pair (2l, 2l+1), so every
 // row moves as one 2-element vector load and store per lane (16 bytes for
-// double).  Streams of 32 rows keep HBM busier than one stream per 128-row
-// plane.  Even waves walk their rows downwards, odd waves upwards, so two
+// double).  Short row streams keep HBM busier than one stream per 128-row
+// plane (measured: 8 waves of 16 rows beat 1 wave of 128 rows by 16 %).
+// Even waves walk their rows downwards, odd waves upwards, so two
 // neighbours reach their shared boundary (and the up-to-5 input rows both
 // read there) at the same time and the second read finds the rows on chip.
 // The Vx window slides with the walk; the six taps are always summed from
@@ -656,8 +662,15 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
 // the pair shares one 7-value read.  A wave's LDS row is written and read by
 // that wave only: the write is waited for (lgkmcnt) before the reads, no
 // workgroup barrier.
-template <typename S, int NVX, int WPP, int PF, bool NT>
+// DEV (deviation storage, S = float): fin/fout hold g = f - feq with feq
+// [NVX][NVY] float32 (constant in x, y; feq is defined as those float32
+// values, so nothing is lost by their storage); the kick evaluates
+// h = feq + g in float64 with float64 weights and stores float(K(h) - feq).
+// feq (64 KB at 128 x 128) is read from L2 twice per row: with the new
+// window row and for the output row.
+template <typename S, int NVX, int WPP, int PF, bool NT, bool DEV>
 __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict__ fin, S* __restrict__ fout,
+                                                             const float* __restrict__ feq,
                                                              const double* __restrict__ Ex,
                                                              const double* __restrict__ Ey, double dt, double dvx,
                                                              double dvy) {
@@ -665,11 +678,15 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     constexpr int PAD = 16;  // >= 10 on the left, >= 12 on the right (taps of l in [-8, NVY + 8])
     constexpr int RPW = NVX / WPP;
     static_assert(NVX % WPP == 0 && RPW % PF == 0, "rows per wave must be a multiple of PF");
-    typedef S V2 __attribute__((ext_vector_type(2)));
-    __shared__ S wx[NVX * 6];
+    static_assert(!DEV || sizeof(S) == 4, "deviation storage is float32");
+    typedef typename std::conditional<DEV, double, S>::type C;  // arithmetic type
+    typedef S V2s __attribute__((ext_vector_type(2)));
+    typedef C V2 __attribute__((ext_vector_type(2)));
+    typedef double V2d __attribute__((ext_vector_type(2)));
+    __shared__ C wx[NVX * 6];
     __shared__ int ilx[NVX];
     __shared__ int irregular[1];
-    __shared__ S rowbuf[WPP][2][NVY + 2 * PAD];
+    __shared__ C rowbuf[WPP][2][NVY + 2 * PAD];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;
@@ -681,16 +698,18 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
         const Departure d = departure(r, ex, dt, dvx);
         double w[6];
         lagrange5_weights<double>(d.alpha, w);
+        C wc[6];
+        round_weights<C>(w, wc);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) wx[r * 6 + k] = static_cast<S>(w[k]);
+        for (int k = 0; k < 6; ++k) wx[r * 6 + k] = wc[k];
         ilx[r] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
     }
     // zero pads of the wave's two row buffers: 2 x 2 x PAD = 64 entries, one per lane
     {
         const int b = lane >> 5, i = lane & 31;
-        rowbuf[wv][b][i < PAD ? i : NVY + i] = S(0);
+        rowbuf[wv][b][i < PAD ? i : NVY + i] = C(0);
     }
-    S wy[2][6];
+    C wy[2][6];
     int ty[2];  // LDS index of tap 0 of each column
     {
         const double ey = Ey[p];
@@ -699,8 +718,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             const Departure d = departure(2 * lane + j, ey, dt, dvy);
             double w[6];
             lagrange5_weights<double>(d.alpha, w);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) wy[j][k] = static_cast<S>(w[k]);
+            round_weights<C>(w, wy[j]);
             const int l = static_cast<int>(d.il < -8 ? -8 : (d.il > NVY + 8 ? NVY + 8 : d.il));
             ty[j] = PAD + l - 2;
         }
@@ -715,23 +733,44 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
 
     const S* src = fin + p * NVX * NVY + 2 * lane;
     S* dst = fout + p * NVX * NVY + 2 * lane;
-    auto ld = [&](int row) -> V2 {
-        const V2* a = reinterpret_cast<const V2*>(src + row * NVY);
-        return NT ? __builtin_nontemporal_load(a) : *a;
+    const float* eq = DEV ? feq + 2 * lane : nullptr;
+    typedef float V2f __attribute__((ext_vector_type(2)));
+    // A row as loaded (g, and feq for DEV) and as used (h = feq + g, or f).
+    struct Raw {
+        V2s g;
+        V2f e;
+    };
+    auto ld = [&](int row) -> Raw {
+        Raw r;
+        const V2s* a = reinterpret_cast<const V2s*>(src + row * NVY);
+        r.g = NT ? __builtin_nontemporal_load(a) : *a;
+        if constexpr (DEV) r.e = *reinterpret_cast<const V2f*>(eq + row * NVY);
+        return r;
+    };
+    auto val = [](const Raw& r) -> V2 {
+        V2 v;
+        if constexpr (DEV) {
+            v.x = static_cast<double>(r.e.x) + static_cast<double>(r.g.x);
+            v.y = static_cast<double>(r.e.y) + static_cast<double>(r.g.y);
+        } else {
+            v = r.g;
+        }
+        return v;
     };
     auto clampr = [](int j) { return j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j); };
-    auto maskr = [](int j) { return (j >= 0 && j < NVX) ? S(1) : S(0); };
-    // Vy kick of row r from the Vx results ox of the lane's two columns.
-    auto vy_row = [&](int r, S ox0, S ox1) {
-        S* rb = rowbuf[wv][r & 1];
+    auto maskr = [](int j) { return (j >= 0 && j < NVX) ? C(1) : C(0); };
+    // Vy kick of row r from the Vx results ox of the lane's two columns;
+    // eo: feq of row r at those columns (DEV).
+    auto vy_row = [&](int r, C ox0, C ox1, V2f eo) {
+        C* rb = rowbuf[wv][r & 1];
         V2 o;
         o.x = ox0;
         o.y = ox1;
         *reinterpret_cast<V2*>(&rb[PAD + 2 * lane]) = o;
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        S a[6], b[6];
+        C a[6], b[6];
         if (pair) {
-            S v[7];
+            C v[7];
 #pragma unroll
             for (int i = 0; i < 7; ++i) v[i] = rb[ty[0] + i];
 #pragma unroll
@@ -746,34 +785,46 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 b[k] = rb[ty[1] + k];
             }
         }
-        V2 res;
-        res.x = tap_sum<S>(wy[0], a);
-        res.y = tap_sum<S>(wy[1], b);
-        V2* out = reinterpret_cast<V2*>(dst + r * NVY);
+        V2s res;
+        if constexpr (DEV) {
+            res.x = static_cast<float>(tap_sum<double>(wy[0], a) - static_cast<double>(eo.x));
+            res.y = static_cast<float>(tap_sum<double>(wy[1], b) - static_cast<double>(eo.y));
+        } else {
+            res.x = tap_sum<C>(wy[0], a);
+            res.y = tap_sum<C>(wy[1], b);
+        }
+        V2s* out = reinterpret_cast<V2s*>(dst + r * NVY);
         if (NT)
             __builtin_nontemporal_store(res, out);
         else
             *out = res;
     };
-    auto vx_pair = [&](int r, const V2 v[6], S& ox0, S& ox1) {
-        S w[6], a[6], b[6];
+    // feq of output row r (DEV; an on-chip read: the 64 KB table stays in L2)
+    auto eq_row = [&](int r) -> V2f {
+        V2f e = {0.0f, 0.0f};
+        if constexpr (DEV) e = *reinterpret_cast<const V2f*>(eq + r * NVY);
+        return e;
+    };
+    auto vx_pair = [&](int r, const V2 v[6], C& ox0, C& ox1) {
+        C w[6], a[6], b[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             w[k] = wx[r * 6 + k];
             a[k] = v[k].x;
             b[k] = v[k].y;
         }
-        ox0 = tap_sum<S>(w, a);
-        ox1 = tap_sum<S>(w, b);
+        ox0 = tap_sum<C>(w, a);
+        ox1 = tap_sum<C>(w, b);
     };
 
     const int r_lo = wv * RPW;
     const bool down = WPP > 1 && (wv & 1) == 0;
     if (regular && !down) {
         // rows r_lo .. r_lo + RPW - 1 upwards; window = taps 0..4 of row r
-        V2 win[5], q[PF];
+        V2 win[5];
+        Raw q[PF];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) win[k] = ld(clampr(r_lo + s0 - 2 + k)) * maskr(r_lo + s0 - 2 + k);
+        for (int k = 0; k < 5; ++k) win[k] = val(ld(clampr(r_lo + s0 - 2 + k))) * maskr(r_lo + s0 - 2 + k);
 #pragma unroll
         for (int u = 0; u < PF; ++u) q[u] = ld(clampr(r_lo + s0 + 3 + u));
         for (int i0 = 0; i0 < RPW; i0 += PF) {
@@ -781,9 +832,10 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             for (int u = 0; u < PF; ++u) {
                 const int r = r_lo + i0 + u;
                 const int jn = r + s0 + 3;  // row held by q[u]: tap 5 of row r
-                const V2 v5 = q[u] * maskr(jn);
+                const V2 v5 = val(q[u]) * maskr(jn);
                 const V2 v[6] = {win[0], win[1], win[2], win[3], win[4], v5};
-                S ox0, ox1;
+                const V2f eo = eq_row(r);
+                C ox0, ox1;
                 vx_pair(r, v, ox0, ox1);
                 win[0] = win[1];
                 win[1] = win[2];
@@ -791,15 +843,16 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 win[3] = win[4];
                 win[4] = v5;
                 if (i0 + u + PF < RPW) q[u] = ld(clampr(jn + PF));
-                vy_row(r, ox0, ox1);
+                vy_row(r, ox0, ox1, eo);
             }
         }
     } else if (regular) {
         // rows r_lo + RPW - 1 .. r_lo downwards; window = taps 1..5 of row r
         const int r_hi = r_lo + RPW - 1;
-        V2 win[5], q[PF];
+        V2 win[5];
+        Raw q[PF];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) win[k] = ld(clampr(r_hi + s0 - 1 + k)) * maskr(r_hi + s0 - 1 + k);
+        for (int k = 0; k < 5; ++k) win[k] = val(ld(clampr(r_hi + s0 - 1 + k))) * maskr(r_hi + s0 - 1 + k);
 #pragma unroll
         for (int u = 0; u < PF; ++u) q[u] = ld(clampr(r_hi + s0 - 2 - u));
         for (int i0 = 0; i0 < RPW; i0 += PF) {
@@ -807,9 +860,10 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             for (int u = 0; u < PF; ++u) {
                 const int r = r_hi - i0 - u;
                 const int jn = r + s0 - 2;  // row held by q[u]: tap 0 of row r
-                const V2 v0 = q[u] * maskr(jn);
+                const V2 v0 = val(q[u]) * maskr(jn);
                 const V2 v[6] = {v0, win[0], win[1], win[2], win[3], win[4]};
-                S ox0, ox1;
+                const V2f eo = eq_row(r);
+                C ox0, ox1;
                 vx_pair(r, v, ox0, ox1);
                 win[4] = win[3];
                 win[3] = win[2];
@@ -817,7 +871,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 win[1] = win[0];
                 win[0] = v0;
                 if (i0 + u + PF < RPW) q[u] = ld(clampr(jn - PF));
-                vy_row(r, ox0, ox1);
+                vy_row(r, ox0, ox1, eo);
             }
         }
     } else {
@@ -826,10 +880,11 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             const int l = ilx[r];
             V2 v[6];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) v[k] = ld(clampr(l + k - 2)) * maskr(l + k - 2);
-            S ox0, ox1;
+            for (int k = 0; k < 6; ++k) v[k] = val(ld(clampr(l + k - 2))) * maskr(l + k - 2);
+            const V2f eo = eq_row(r);
+            C ox0, ox1;
             vx_pair(r, v, ox0, ox1);
-            vy_row(r, ox0, ox1);
+            vy_row(r, ox0, ox1, eo);
         }
     }
 }
@@ -1091,22 +1146,31 @@ static int launch_kick_stream(const S* fin, S* fout, int nplanes, const double* 
 }
 
 // Row-split kick, 128 vy columns, WPP waves per plane, pf8: 8 rows
-// prefetched (else 4), at most the rows per wave.
+// prefetched (else 4), at most the rows per wave; feq != nullptr selects
+// deviation storage (float only).
 template <typename S, int NVX, int WPP>
-static int launch_kick_rows(bool pf8, const S* fin, S* fout, int nplanes, const double* Ex, const double* Ey,
-                            double dt, double dvx, double dvy, hipStream_t s) {
+static int launch_kick_rows(bool pf8, const S* fin, S* fout, const float* feq, int nplanes, const double* Ex,
+                            const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
     constexpr int RPW = NVX / WPP;
     constexpr int PF8 = RPW < 8 ? RPW : 8;
     constexpr int PF4 = RPW < 4 ? RPW : 4;
+    constexpr bool F32 = sizeof(S) == 4;
     const bool nt = (nt_mask() & 4) != 0;
-#define VP_K(PF, NT)                                                                                               \
-    hipLaunchKernelGGL((kick_rows_kernel<S, NVX, WPP, PF, NT>), dim3(nplanes), dim3(64 * WPP), 0, s, fin, fout, Ex, \
-                       Ey, dt, dvx, dvy)
-    if (pf8) {
-        if (nt) VP_K(PF8, true); else VP_K(PF8, false);
+#define VP_K(PF, NT, DEV)                                                                                     \
+    hipLaunchKernelGGL((kick_rows_kernel<S, NVX, WPP, PF, NT, DEV>), dim3(nplanes), dim3(64 * WPP), 0, s, fin, \
+                       fout, feq, Ex, Ey, dt, dvx, dvy)
+#define VP_K2(PF, DEV)                      \
+    if (nt) VP_K(PF, true, DEV); else VP_K(PF, false, DEV)
+    if (feq) {
+        if constexpr (F32) {
+            if (pf8) { VP_K2(PF8, true); } else { VP_K2(PF4, true); }
+        } else {
+            return VP_ERR_UNSUPPORTED;
+        }
     } else {
-        if (nt) VP_K(PF4, true); else VP_K(PF4, false);
+        if (pf8) { VP_K2(PF8, false); } else { VP_K2(PF4, false); }
     }
+#undef VP_K2
 #undef VP_K
     VP_CHECK_LAUNCH();
     return VP_OK;
@@ -1117,15 +1181,20 @@ static int launch_kick_rows(bool pf8, const S* fin, S* fout, int nplanes, const 
 // row-streaming kernel; the kick-prefetch knob 1 or 3 picks 4 or 8 rows
 // prefetched.  Automatic (knobs 0 and 2): 8 waves per plane, 4 rows
 // prefetched for double and 8 for float (measured fastest at 128^4).
+// Deviation storage always takes a row-split kernel: automatic = 2 waves
+// per plane and 4 rows prefetched (float64 arithmetic: ~125 registers).
 template <typename S, int NVX>
-static int kick_rows_dispatch(int kk, int pfk, const S* fin, S* fout, int nplanes, const double* Ex,
-                              const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
+static int kick_rows_dispatch(int kk, int pfk, const S* fin, S* fout, const float* feq, int nplanes,
+                              const double* Ex, const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
+    if (feq && (kk < 3 || kk > 6)) kk = 3;
     if (kk == 0) kk = 5;
-    const bool pf8 = pfk == 3 || (pfk != 1 && sizeof(S) == 4);
-    if (kk == 3) return launch_kick_rows<S, NVX, 2>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
-    if (kk == 4) return launch_kick_rows<S, NVX, 4>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
-    if (kk == 5) return launch_kick_rows<S, NVX, 8>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
-    if (kk == 6 && NVX >= 64) return launch_kick_rows<S, NVX, 16>(pf8, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+    const bool pf8 = pfk == 3 || (pfk != 1 && sizeof(S) == 4 && !feq);
+#define VP_R(W) return launch_kick_rows<S, NVX, W>(pf8, fin, fout, feq, nplanes, Ex, Ey, dt, dvx, dvy, s)
+    if (kk == 3) VP_R(2);
+    if (kk == 4) VP_R(4);
+    if (kk == 6 && NVX >= 64) VP_R(16);
+    if (kk == 5 || kk == 6) VP_R(8);
+#undef VP_R
     return VP_ERR_UNSUPPORTED;  // row-streaming kernel
 }
 
@@ -1137,9 +1206,11 @@ static int kick_stream(const S* fin, S* fout, const int64_t shape[4], const doub
     if (nvy == 128) {
         const int kk = knob(VP_KNOB_KICK_KERNEL), pfk = knob(VP_KNOB_KICK_PREFETCH);
         int rc = VP_ERR_UNSUPPORTED;
-        if (nvx == 128) rc = kick_rows_dispatch<S, 128>(kk, pfk, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
-        if (nvx == 64) rc = kick_rows_dispatch<S, 64>(kk, pfk, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
-        if (nvx == 32) rc = kick_rows_dispatch<S, 32>(kk, pfk, fin, fout, nplanes, Ex, Ey, dt, dvx, dvy, s);
+#define VP_RD(N) rc = kick_rows_dispatch<S, N>(kk, pfk, fin, fout, nullptr, nplanes, Ex, Ey, dt, dvx, dvy, s)
+        if (nvx == 128) VP_RD(128);
+        if (nvx == 64) VP_RD(64);
+        if (nvx == 32) VP_RD(32);
+#undef VP_RD
         if (rc != VP_ERR_UNSUPPORTED) return rc;
     }
 #define VP_KS(A, B) \
@@ -1182,6 +1253,24 @@ int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const d
     VP_K(32, 64)
     VP_K(32, 32)
 #undef VP_K
+    return VP_ERR_UNSUPPORTED;
+}
+
+// FP32 deviation storage: g = f - feq in float32 (feq float32), kick evaluated in float64
+// (row-split kernel; 128 vy columns; g_in and g_out must not overlap).
+int fast2d_kick_dev(const float* gin, float* gout, const int64_t shape[4], const float* feq, const double* Ex,
+                    const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
+    const int64_t nvx = shape[2], nvy = shape[3];
+    const int64_t cells = shape[0] * shape[1] * nvx * nvy;
+    const bool disjoint = gout + cells <= gin || gin + cells <= gout;
+    if (!disjoint || nvy != 128 || !feq) return VP_ERR_UNSUPPORTED;
+    const int nplanes = static_cast<int>(shape[0] * shape[1]);
+    const int kk = knob(VP_KNOB_KICK_KERNEL), pfk = knob(VP_KNOB_KICK_PREFETCH);
+#define VP_RD(N) return kick_rows_dispatch<float, N>(kk, pfk, gin, gout, feq, nplanes, Ex, Ey, dt, dvx, dvy, s)
+    if (nvx == 128) VP_RD(128);
+    if (nvx == 64) VP_RD(64);
+    if (nvx == 32) VP_RD(32);
+#undef VP_RD
     return VP_ERR_UNSUPPORTED;
 }
 

@@ -46,6 +46,7 @@ __global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restric
 // part order) times dvx, then dvy.
 __global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ rho, const double2* __restrict__ partial,
                                                     int nparts, double dvx, double dvy, int nx, int ny,
+                                                    const double* __restrict__ rho_shift,
                                                     double* __restrict__ delta_rho, double2* __restrict__ R) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double2* tw = reinterpret_cast<double2*>(smem_raw);
@@ -63,7 +64,9 @@ __global__ __launch_bounds__(256) void p2d_rows_fwd(const double* __restrict__ r
         } else {
             rv = rho[static_cast<int64_t>(x) * ny + y];
         }
-        const double d = rv - 1.0;
+        // delta_rho = rho - 1; with deviation storage the partial sums hold
+        // sum g dv and rho_shift[0] = sum feq dv - 1
+        const double d = rho_shift ? rv + rho_shift[0] : rv - 1.0;
         row[y] = d;
         if (delta_rho) delta_rho[static_cast<int64_t>(x) * ny + y] = d;
     }
@@ -209,7 +212,8 @@ size_t poisson2d_ws(int64_t nx, int64_t ny) {
 
 int poisson2d_from(const double* rho, const double2* partial, int nparts, double dvx, double dvy, int64_t nx,
                    int64_t ny, const double* kx, const double* ky, const double* inv_ksq, double* Ex, double* Ey,
-                   double* delta_rho, double* phi, void* ws, size_t ws_bytes, hipStream_t s) {
+                   double* delta_rho, double* phi, void* ws, size_t ws_bytes, hipStream_t s,
+                   const double* rho_shift) {
     VP_RETURN_IF((!rho && !partial) || !kx || !ky || !inv_ksq || !ws, VP_ERR_ARG);
     VP_RETURN_IF(nx < 1 || ny < 1 || nx > 512 || ny > 512, VP_ERR_SHAPE);
     VP_RETURN_IF(ws_bytes < poisson2d_ws(nx, ny), VP_ERR_WORKSPACE);
@@ -218,7 +222,7 @@ int poisson2d_from(const double* rho, const double2* partial, int nparts, double
     double2* R = static_cast<double2*>(ws);
     double2* T = R + static_cast<int64_t>(nx) * nky;
     hipLaunchKernelGGL(p2d_rows_fwd, dim3(nxi), dim3(256), ny * (sizeof(double2) + sizeof(double)), s, rho, partial,
-                       nparts, dvx, dvy, nxi, nyi, delta_rho, R);
+                       nparts, dvx, dvy, nxi, nyi, rho_shift, delta_rho, R);
     VP_CHECK_LAUNCH();
     hipLaunchKernelGGL(p2d_cols, dim3(nky), dim3(256), 5 * nx * sizeof(double2), s, R, nxi, nyi, kx, ky,
                        inv_ksq, T);
@@ -234,7 +238,7 @@ int poisson2d(const double* rho, int64_t nx, int64_t ny, const double* kx, const
               size_t ws_bytes, hipStream_t s) {
     VP_RETURN_IF(!rho, VP_ERR_ARG);
     return poisson2d_from(rho, nullptr, 0, 1.0, 1.0, nx, ny, kx, ky, inv_ksq, Ex, Ey, delta_rho, phi, ws, ws_bytes,
-                          s);
+                          s, nullptr);
 }
 
 // out[0] = ((0.5 * sum(a^2 + b^2)) * s1) * s2  (b may be NULL), compensated.

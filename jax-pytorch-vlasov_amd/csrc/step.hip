@@ -17,7 +17,8 @@ int poisson2d(const double* rho, int64_t nx, int64_t ny, const double* kx, const
 
 int poisson2d_from(const double* rho, const double2* partial, int nparts, double dvx, double dvy, int64_t nx,
                    int64_t ny, const double* kx, const double* ky, const double* inv_ksq, double* Ex, double* Ey,
-                   double* delta_rho, double* phi, void* ws, size_t ws_bytes, hipStream_t s);
+                   double* delta_rho, double* phi, void* ws, size_t ws_bytes, hipStream_t s,
+                   const double* rho_shift = nullptr);
 int field_energy(const double* a, const double* b, int64_t n, double s1, double s2, double* out, hipStream_t s);
 
 static inline size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
@@ -208,11 +209,13 @@ extern "C" int vp_fused_kick_f64(const double* f_in, double* f_out, const int64_
 // nsteps Strang steps in place on f.  Between two steps the closing X(dt/2)
 // and the opening X(dt/2) run in one pass (applied twice in LDS): the
 // result is identical to nsteps separate steps.
+// feq / rho_shift (float only): deviation storage, f holds g = f - feq.
 template <typename S>
 static int run_2d2v(S* f, const int64_t shape[4], int nsteps, double dt, const double* vx, const double* vy,
                     const double* kx, const double* ky, const double* inv_ksq, double dx, double dy, double dvx,
                     double dvy, int physics_mode, double* Ex, double* Ey, double* delta_rho, double* phi,
-                    double* energy, void* ws, hipStream_t s) {
+                    double* energy, void* ws, hipStream_t s, const float* feq = nullptr,
+                    const double* rho_shift = nullptr) {
     const int64_t nx = shape[0], ny = shape[1];
     vp::Fast2DWs L = vp::fast2d_layout(shape, ws, static_cast<int>(sizeof(S)), true);
     void* pws = L.pws;
@@ -230,9 +233,16 @@ static int run_2d2v(S* f, const int64_t shape[4], int nsteps, double dt, const d
         VP_TRY(vp::fast2d_ydrift(f, f, shape, L, physics_mode != 0, s));
         if (physics_mode)
             VP_TRY(vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, nx, ny, kx, ky, inv_ksq, Ex, Ey,
-                                      delta_rho, phi, pws, vp::poisson2d_ws(nx, ny), s));
+                                      delta_rho, phi, pws, vp::poisson2d_ws(nx, ny), s, rho_shift));
         if (energy) VP_TRY(vp::field_energy(Ex, Ey, nx * ny, dx, dy, energy + st, s));
-        VP_TRY(vp::fast2d_kick(f, tmp, shape, Ex, Ey, dt, dvx, dvy, s));
+        if constexpr (sizeof(S) == 4) {
+            if (feq)
+                VP_TRY(vp::fast2d_kick_dev(f, tmp, shape, feq, Ex, Ey, dt, dvx, dvy, s));
+            else
+                VP_TRY(vp::fast2d_kick(f, tmp, shape, Ex, Ey, dt, dvx, dvy, s));
+        } else {
+            VP_TRY(vp::fast2d_kick(f, tmp, shape, Ex, Ey, dt, dvx, dvy, s));
+        }
         VP_TRY(vp::fast2d_ydrift(tmp, f, shape, L, false, s));
         VP_TRY(vp::fast2d_xdrift(f, f, shape, L, st == nsteps - 1 ? 1 : 2, s));
     }
@@ -265,6 +275,44 @@ extern "C" int vp_run_2d2v_f32(float* f, const int64_t shape[4], int nsteps, dou
     VP_RUN_CHECKS(vp::fast2d_supported_f32(shape), fused_2d_ws(shape, 4, true));
     return run_2d2v<float>(f, shape, nsteps, dt, vx, vy, kx, ky, inv_ksq, dx, dy, dvx, dvy, physics_mode, Ex, Ey,
                            delta_rho, phi, energy, ws, static_cast<hipStream_t>(stream));
+}
+
+// FP32 deviation storage: g (float32) = f - feq, feq [nvx][nvy] float32,
+// rho_shift[0] = sum feq dvx dvy - 1 (both from vp_dev_split_f32).
+static bool dev_supported(const int64_t shape[4]) {
+    return shape && vp::fast2d_supported_f32(shape) && shape[3] == 128;
+}
+
+extern "C" int vp_run_2d2v_dev_f32(float* g, const float* feq, const double* rho_shift, const int64_t shape[4],
+                                   int nsteps, double dt, const double* vx, const double* vy, const double* kx,
+                                   const double* ky, const double* inv_ksq, double dx, double dy, double dvx,
+                                   double dvy, int physics_mode, double* Ex, double* Ey, double* delta_rho,
+                                   double* phi, double* energy, void* ws, size_t ws_bytes, vp_stream_t stream) {
+    float* f = g;
+    VP_RETURN_IF(!feq || !rho_shift, VP_ERR_ARG);
+    VP_RUN_CHECKS(dev_supported(shape), fused_2d_ws(shape, 4, true));
+    return run_2d2v<float>(f, shape, nsteps, dt, vx, vy, kx, ky, inv_ksq, dx, dy, dvx, dvy, physics_mode, Ex, Ey,
+                           delta_rho, phi, energy, ws, static_cast<hipStream_t>(stream), feq, rho_shift);
+}
+
+extern "C" int vp_fused_poisson_dev_f32(const int64_t shape[4], double dvx, double dvy, const double* rho_shift,
+                                        const double* kx, const double* ky, const double* inv_ksq, double* Ex,
+                                        double* Ey, double* delta_rho, double* phi, void* ws, size_t ws_bytes,
+                                        vp_stream_t stream) {
+    VP_FUSED_GUARD_F32(ws, ws_bytes);
+    VP_RETURN_IF(!rho_shift, VP_ERR_ARG);
+    vp::Fast2DWs L = vp::fast2d_layout(shape, ws, 4, false);
+    return vp::poisson2d_from(nullptr, L.partial, L.nparts, dvx, dvy, shape[0], shape[1], kx, ky, inv_ksq, Ex, Ey,
+                              delta_rho, phi, L.pws, L.pws_bytes, static_cast<hipStream_t>(stream), rho_shift);
+}
+
+extern "C" int vp_fused_kick_dev_f32(const float* g_in, float* g_out, const int64_t shape[4], const float* feq,
+                                     const double* Ex, const double* Ey, double dt, double dvx, double dvy,
+                                     vp_stream_t stream) {
+    VP_RETURN_IF(!shape || !g_in || !g_out || !feq || !Ex || !Ey, VP_ERR_ARG);
+    VP_RETURN_IF(shape[0] < 1 || shape[1] < 1 || shape[0] * shape[1] > 0x7fffffffLL, VP_ERR_SHAPE);
+    VP_RETURN_IF(!vp::fast2d_kick_supported(shape) || shape[3] != 128, VP_ERR_UNSUPPORTED);
+    return vp::fast2d_kick_dev(g_in, g_out, shape, feq, Ex, Ey, dt, dvx, dvy, static_cast<hipStream_t>(stream));
 }
 #undef VP_RUN_CHECKS
 

@@ -85,6 +85,41 @@ __device__ __forceinline__ T tap_sum(const T w[6], const T v[6]) {
     return w[0] * v[0] + w[1] * v[1] + w[2] * v[2] + w[3] * v[3] + w[4] * v[4] + w[5] * v[5];
 }
 
+// Float64 weights in the storage type S of a sweep.  double: as computed.
+// float (FP32 storage): each weight rounded to nearest, then the one of
+// smallest magnitude replaced by 1 minus the (exact, float64) sum of the
+// other five, rounded to float, so the six float weights sum to 1 within
+// half an ulp of the smallest weight instead of within a few ulp of the
+// largest.  Without that the rounding bias of the weights (~1e-7) scales f
+// by a velocity-dependent factor in every sweep, a systematic drift that
+// the field energy integrates over hundreds of steps.
+template <typename S>
+__device__ __forceinline__ void round_weights(const double w[6], S out[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = static_cast<S>(w[k]);
+}
+
+template <>
+__device__ __forceinline__ void round_weights<float>(const double w[6], float out[6]) {
+    int m = 0;
+    double am = fabs(w[0]);
+#pragma unroll
+    for (int k = 1; k < 6; ++k) {
+        const bool smaller = fabs(w[k]) < am;
+        am = smaller ? fabs(w[k]) : am;
+        m = smaller ? k : m;
+    }
+    double rest = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        out[k] = static_cast<float>(w[k]);
+        rest += (k == m) ? 0.0 : static_cast<double>(out[k]);
+    }
+    const float bal = static_cast<float>(1.0 - rest);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = (k == m) ? bal : out[k];
+}
+
 // ---------------------------------------------------------------------------
 // Neumaier compensated summation.
 // ---------------------------------------------------------------------------

@@ -63,6 +63,14 @@ _SIGS = {
     "vp_fused_poisson_f32": (c_int, [I64x4, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                      c_sz, c_vp]),
     "vp_fused_kick_f32": (c_int, [c_vp, c_vp, I64x4, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp]),
+    "vp_dev_split_workspace": (c_sz, [I64x4]),
+    "vp_dev_split_f32": (c_int, [c_vp, I64x4, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "vp_dev_join_f32": (c_int, [c_vp, c_vp, I64x4, c_vp, c_vp, c_vp]),
+    "vp_run_2d2v_dev_f32": (c_int, [c_vp, c_vp, c_vp, I64x4, c_int, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl,
+                                    c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "vp_fused_kick_dev_f32": (c_int, [c_vp, c_vp, I64x4, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_vp]),
+    "vp_fused_poisson_dev_f32": (c_int, [I64x4, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp, c_sz, c_vp]),
     "vp_field_energy_f64": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
     "vp_moments_workspace": (c_sz, []),
     "vp_moments_f64": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp, c_sz, c_vp]),

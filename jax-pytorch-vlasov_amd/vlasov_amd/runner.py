@@ -14,7 +14,10 @@ them step after step without allocating.  Two kernel paths:
 For float64 f both paths give the same f as repeated calls of
 ``vlasov_step_impl``.  float32 f is the FP32-storage variant: f stored and
 interpolated in float32, charge density, Poisson solve, fields and field
-energy in float64.
+energy in float64.  ``storage="deviation"`` (fused, nvy = 128) stores
+g = f - feq in float32 instead, feq the (x, y) mean of the initial
+f: the drifts run on g, the kick on feq + g in float64, and storage
+rounding scales with the perturbation instead of with f.
 """
 from __future__ import annotations
 
@@ -36,6 +39,11 @@ def fused_supported(shape, dtype=torch.float64) -> bool:
     return q(_lib.i64x4(shape)) > 0
 
 
+def deviation_supported(shape) -> bool:
+    """True when FP32 deviation storage runs for this f shape."""
+    return len(shape) == 4 and int(shape[3]) == 128 and fused_supported(shape, torch.float32)
+
+
 def _suffix(dtype) -> str:
     if dtype == torch.float64:
         return "f64"
@@ -48,19 +56,42 @@ class Run2D:
     """In-place 2D2V Strang stepping of f[Nx, Ny, Nvx, Nvy] (float64, or
     float32 for the FP32-storage variant)."""
 
-    def __init__(self, grid, f: torch.Tensor, dt: float, physics_mode: bool = True, fused: bool | None = None):
+    def __init__(self, grid, f: torch.Tensor, dt: float, physics_mode: bool = True, fused: bool | None = None,
+                 storage: str = "plain"):
         self.g = grid
         self.dt = float(dt)
         self.physics_mode = bool(physics_mode)
         dev = f.device
         self.device = dev
         self.shape = tuple(int(n) for n in f.shape)
-        self.sfx = _suffix(f.dtype)
-        self.fused = fused_supported(self.shape, f.dtype) if fused is None else bool(fused)
-        if self.fused and not fused_supported(self.shape, f.dtype):
-            raise ValueError(f"no fused kernels for shape {self.shape} and {f.dtype}")
+        if storage not in ("plain", "deviation"):
+            raise ValueError(f"storage must be 'plain' or 'deviation', got {storage!r}")
+        self.deviation = storage == "deviation"
+        if self.deviation:
+            if not deviation_supported(self.shape) or fused is False:
+                raise ValueError(f"FP32 deviation storage needs fused float32 kernels and nvy = 128, got {self.shape}")
+            self.sfx = "f32"
+            self.fused = True
+        else:
+            self.sfx = _suffix(f.dtype)
+            self.fused = fused_supported(self.shape, f.dtype) if fused is None else bool(fused)
+            if self.fused and not fused_supported(self.shape, f.dtype):
+                raise ValueError(f"no fused kernels for shape {self.shape} and {f.dtype}")
         Nx, Ny = self.shape[0], self.shape[1]
-        self.f = [f.contiguous().clone(), torch.empty_like(f)]
+        if self.deviation:
+            # split on the device: feq = (x, y) mean of f (float64), g = float32(f - feq)
+            f64 = f.contiguous() if f.dtype == torch.float64 else f.contiguous().to(torch.float64)
+            sh = _lib.i64x4(self.shape)
+            self.feq = torch.empty(self.shape[2:], dtype=torch.float32, device=dev)
+            self.rho_shift = torch.empty(1, dtype=torch.float64, device=dev)
+            g = torch.empty(self.shape, dtype=torch.float32, device=dev)
+            sws = torch.empty(lib().vp_dev_split_workspace(sh), dtype=torch.uint8, device=dev)
+            check(lib().vp_dev_split_f32(f64.data_ptr(), sh, float(grid.dvx), float(grid.dvy), self.feq.data_ptr(),
+                                         self.rho_shift.data_ptr(), g.data_ptr(), sws.data_ptr(), sws.numel(),
+                                         stream_handle(dev)), "vp_dev_split_f32")
+            self.f = [g, torch.empty_like(g)]
+        else:
+            self.f = [f.contiguous().clone(), torch.empty_like(f)]
         self.fields = torch.zeros((4, Nx, Ny), dtype=torch.float64, device=dev)  # Ex, Ey, delta_rho, phi
         f64 = torch.float64
         self.vx = grid.vx.to(f64).contiguous()
@@ -84,7 +115,23 @@ class Run2D:
 
     @property
     def state(self) -> torch.Tensor:
+        """f: the stored tensor, or feq + g rounded to float32 (a new tensor) with
+        deviation storage."""
+        if self.deviation:
+            out = torch.empty(self.shape, dtype=torch.float32, device=self.device)
+            check(lib().vp_dev_join_f32(self.f[0].data_ptr(), self.feq.data_ptr(), _lib.i64x4(self.shape),
+                                        out.data_ptr(), None, stream_handle(self.device)), "vp_dev_join_f32")
+            return out
         return self.f[0]
+
+    def state_f64(self) -> torch.Tensor:
+        """f in float64 (feq + g exactly with deviation storage)."""
+        if not self.deviation:
+            return self.f[0].to(torch.float64)
+        out = torch.empty(self.shape, dtype=torch.float64, device=self.device)
+        check(lib().vp_dev_join_f32(self.f[0].data_ptr(), self.feq.data_ptr(), _lib.i64x4(self.shape), None,
+                                    out.data_ptr(), stream_handle(self.device)), "vp_dev_join_f32")
+        return out
 
     @property
     def Ex(self):
@@ -137,8 +184,17 @@ class Run2D:
         tables = getattr(L, f"vp_fused_tables_{sfx}")
         xdrift = getattr(L, f"vp_fused_xdrift_{sfx}")
         ydrift = getattr(L, f"vp_fused_ydrift_{sfx}")
-        poisson = getattr(L, f"vp_fused_poisson_{sfx}")
-        kick = getattr(L, f"vp_fused_kick_{sfx}")
+        if self.deviation:
+            feq, rsh = self.feq.data_ptr(), self.rho_shift.data_ptr()
+
+            def poisson(sh, dvx, dvy, kx, ky, iks, Ex, Ey, drho, phi, ws, wsn, s):
+                return L.vp_fused_poisson_dev_f32(sh, dvx, dvy, rsh, kx, ky, iks, Ex, Ey, drho, phi, ws, wsn, s)
+
+            def kick(fin, fout, sh, Ex, Ey, dt, dvx, dvy, s):
+                return L.vp_fused_kick_dev_f32(fin, fout, sh, feq, Ex, Ey, dt, dvx, dvy, s)
+        else:
+            poisson = getattr(L, f"vp_fused_poisson_{sfx}")
+            kick = getattr(L, f"vp_fused_kick_{sfx}")
         fp = self.f[0].data_ptr()
         fp2 = self.f[1].data_ptr()  # the kick writes here; the closing Y drift reads it back
         ws, wsn = self.ws.data_ptr(), self.ws.numel()

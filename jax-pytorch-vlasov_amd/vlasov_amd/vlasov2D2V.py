@@ -406,19 +406,29 @@ class Vlasov:
 
     def __init__(self, Nx=32, Ny=32, Nvx=32, Nvy=32, Lx=2.0 * math.pi, Ly=2.0 * math.pi, Vx_max=6.0,
                  Vy_max=6.0, nbiter=10, diag_steps=10, dt=0.05, out_dir="data", physics_mode=False,
-                 epsilon=0.001, dtype: str = "float64", solver: int = 0, device=None):
+                 epsilon=0.001, dtype: str = "float64", solver: int = 0, device=None,
+                 fp32_storage: str = "plain"):
         """Same parameters as the reference constructor (P2:1236-1314).
 
-        ``solver`` keeps the reference ids for the log; every id runs the HIP
-        step, which follows solver 0's arithmetic (vlasov_step_impl).
+        ``solver`` selects the step as the reference's ``solver_fns`` does:
+        0 vlasov_step_impl, 1 vlasov_step_matrix_diag, 2
+        vlasov_step_gather_free (all on the library's kernels).
         dtype='float32' is the FP32-storage variant: f is float32, the grid,
         charge density, Poisson solve and fields stay float64 (the
         reference's float32 mode computes everything in float32).
+        ``fp32_storage`` (float32 only): "plain" stores f itself; "deviation"
+        stores g = f - feq in float32, feq the (x, y) mean of the initial f in
+        float32 (used when the shape has those kernels: nvy = 128), which cuts
+        the storage error of the field energy ~20x for a slower kick.
         """
         self.output_log = ""
         if dtype not in ("float64", "float32"):
             raise ValueError(f"dtype must be 'float64' or 'float32', got {dtype!r}")
         self.f_dtype = torch.float64 if dtype == "float64" else torch.float32
+        if fp32_storage not in ("deviation", "plain"):
+            raise ValueError(f"fp32_storage must be 'deviation' or 'plain', got {fp32_storage!r}")
+        self.fp32_storage = fp32_storage
+        self._f_init64 = None  # float64 initial f, split exactly by deviation storage
         self.dtype = torch.float64
         self.device = torch.device(device) if device is not None else default_device()
         self.grid = Grid.create(Nx, Ny, Nvx, Nvy, Lx, Ly, Vx_max, Vy_max, device=self.device, dtype=self.dtype)
@@ -489,6 +499,8 @@ class Vlasov:
             vy2 = (vy_mesh[None, None, :, :] - 0.0) ** 2 / sigma_v ** 2
             f_data = torch.exp(-0.5 * (dx2 + dy2 + vx2 + vy2))
             f0 = torch.zeros((g.Nvx, g.Nvy), device=self.device, dtype=self.dtype)
+        if self.f_dtype == torch.float32:
+            self._f_init64 = f_data.to(torch.float64).contiguous()
         f_data = f_data.to(self.f_dtype).contiguous()
         *_, rho, phi = solve_poisson_real(f_data, g.kx, g.ky, g.inv_ksq, g.dvx, g.dvy)
         state = SimulationState(field=Field(rho, phi), variables=Variables(fn=f_data, f0=f0), t=0.0, step=0)
@@ -528,7 +540,14 @@ class Vlasov:
         synchronize(self.device)
         if self.solver != 0:
             return self._run_steps(state, energy)
-        runner = Run2D(g, f, self.dt, self.physics_mode)
+        from .runner import deviation_supported
+        if (self.f_dtype == torch.float32 and self.fp32_storage == "deviation"
+                and deviation_supported(tuple(f.shape))):
+            # split the exact float64 initial f when this run starts from it
+            f_split = self._f_init64 if (state.step == 0 and self._f_init64 is not None) else f
+            runner = Run2D(g, f_split, self.dt, self.physics_mode, storage="deviation")
+        else:
+            runner = Run2D(g, f, self.dt, self.physics_mode)
         done = 0
         while done < self.nbiter:
             chunk = min(self.diag_steps, self.nbiter - done)

@@ -170,3 +170,126 @@ def test_fp32_kick_kernels_agree(device, shape):
     a = v2.advect_1d(f, Ex[:, :, None], 2, dt, dvx, periodic=False)
     a = v2.advect_1d(a, Ey[:, :, None], 3, dt, dvy, periodic=False)
     assert torch.equal(outs[0], a)
+
+
+# ---------------------------------------------------------------------------
+# Deviation storage: float32 g = f - feq, feq the float64 (x, y) mean of f.
+# ---------------------------------------------------------------------------
+def _split(L, f64, shape, dvx, dvy):
+    lib = L.lib()
+    dev = f64.device
+    sh = L.i64x4(shape)
+    feq = torch.empty(shape[2:], dtype=torch.float32, device=dev)
+    rs = torch.empty(1, dtype=torch.float64, device=dev)
+    g = torch.empty(shape, dtype=torch.float32, device=dev)
+    ws = torch.empty(lib.vp_dev_split_workspace(sh), dtype=torch.uint8, device=dev)
+    s = torch.cuda.current_stream().cuda_stream
+    L.check(lib.vp_dev_split_f32(f64.data_ptr(), sh, dvx, dvy, feq.data_ptr(), rs.data_ptr(), g.data_ptr(),
+                                 ws.data_ptr(), ws.numel(), s), "vp_dev_split_f32")
+    return feq, rs, g
+
+
+def test_deviation_split_and_join(device):
+    v2, runner, L = _mods()
+    shape = (8, 6, 32, 128)
+    rng = np.random.default_rng(21)
+    f = rng.random(shape) + 1.0
+    dvx, dvy = 0.3, 0.2
+    feq, rs, g = _split(L, torch.from_numpy(f).to(device), shape, dvx, dvy)
+    ref_feq = f.reshape(-1, shape[2] * shape[3]).mean(axis=0).reshape(shape[2:])
+    e = feq.cpu().numpy().astype(np.float64)  # feq is defined as these float32 values
+    np.testing.assert_allclose(e, ref_feq, rtol=2.0 ** -24)
+    assert abs(float(rs) - ((e.sum() * dvx) * dvy - 1.0)) <= 1e-12 * e.sum() * dvx * dvy
+    np.testing.assert_array_equal(g.cpu().numpy(), (f - e[None, None]).astype(np.float32))
+    out32 = torch.empty(shape, dtype=torch.float32, device=device)
+    out64 = torch.empty(shape, dtype=torch.float64, device=device)
+    L.check(L.lib().vp_dev_join_f32(g.data_ptr(), feq.data_ptr(), L.i64x4(shape), out32.data_ptr(), out64.data_ptr(),
+                                    torch.cuda.current_stream().cuda_stream), "vp_dev_join_f32")
+    h = feq.cpu().numpy().astype(np.float64)[None, None] + g.cpu().numpy().astype(np.float64)
+    np.testing.assert_array_equal(out64.cpu().numpy(), h)
+    np.testing.assert_array_equal(out32.cpu().numpy(), h.astype(np.float32))
+    # the rounding of the stored deviation only: |f - join| <= half an ulp of g
+    assert np.max(np.abs(out64.cpu().numpy() - f)) <= np.max(np.abs(f - e[None, None])) * 2.0 ** -24 * 1.01
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 128, 128), (4, 8, 32, 128)])
+def test_deviation_kick_equals_float64_kick_of_the_sum(device, shape):
+    """The deviation kick stores float32(K(feq + g) - feq) with K the float64
+    kick: bit for bit the float64 oracle kick of the same feq + g, for every
+    launch shape (the output feq from the Vx window or read; multi-cell and
+    out-of-domain shifts)."""
+    v2, runner, L = _mods()
+    lib = L.lib()
+    nx, ny, nvx, nvy = shape
+    rng = np.random.default_rng(8)
+    feq = np.exp(-0.5 * np.linspace(-4, 4, nvx)[:, None] ** 2
+                 - 0.5 * np.linspace(-4, 4, nvy)[None, :] ** 2).astype(np.float32)
+    g = (rng.random(shape) - 0.5).astype(np.float32) * np.float32(1e-3)
+    dt, dvx, dvy = 0.1, 8.0 / (nvx - 1), 8.0 / (nvy - 1)
+    choices = np.array([0.0, 1e-18, 0.4, -0.7, 2.0, 5.3, -5.3, 80.0, -80.0])
+    Ex = rng.choice(choices, size=(nx, ny)) * dvx / dt
+    Ey = rng.choice(choices, size=(nx, ny)) * dvy / dt
+    h = feq.astype(np.float64)[None, None] + g.astype(np.float64)
+    ref = vo.advect_axis(h, Ex[:, :, None], 2, dt, dvx, periodic=False)
+    ref = vo.advect_axis(ref, Ey[:, :, None], 3, dt, dvy, periodic=False)
+    expect = (ref - feq.astype(np.float64)[None, None]).astype(np.float32)
+    tg = torch.from_numpy(g).to(device)
+    out = torch.empty_like(tg)
+    s = torch.cuda.current_stream().cuda_stream
+    args = (L.i64x4(shape), torch.from_numpy(feq).to(device), torch.from_numpy(Ex).to(device),
+            torch.from_numpy(Ey).to(device))
+    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("kick_kernel", "kick_prefetch")}
+    try:
+        for kk, pf in ((0, 2), (0, 1), (4, 3), (3, 1)):
+            lib.vp_tune_set(L.KNOBS["kick_kernel"], kk)
+            lib.vp_tune_set(L.KNOBS["kick_prefetch"], pf)
+            L.check(lib.vp_fused_kick_dev_f32(tg.data_ptr(), out.data_ptr(), args[0], args[1].data_ptr(),
+                                              args[2].data_ptr(), args[3].data_ptr(), dt, dvx, dvy, s),
+                    "vp_fused_kick_dev_f32")
+            np.testing.assert_array_equal(out.cpu().numpy(), expect)
+    finally:
+        for k, v in old.items():
+            lib.vp_tune_set(L.KNOBS[k], v)
+
+
+def test_deviation_trajectory_vs_fp64_oracle(device):
+    """Landau run at 64 x 64 x 32 x 128, 10 steps: the deviation form tracks
+    the float64 oracle to 1e-6 in the field-energy history (the plain
+    float32 form only to 1e-5 here), and mass to 1e-9."""
+    v2, runner, _ = _mods()
+    shape = (64, 64, 32, 128)
+    G, f0 = _setup(shape, 4)
+    g = vo.grid_2d(*shape, 4 * math.pi, 4 * math.pi, 6.0, 6.0)
+    arrays = tuple(a.cpu().numpy() for a in (G.vx, G.vy, G.kx, G.ky, G.inv_ksq))
+    run = runner.Run2D(G, torch.from_numpy(f0).to(device), 0.1, physics_mode=True, storage="deviation")
+    energy = torch.zeros(10, dtype=torch.float64, device=device)
+    run.advance(10, energy=energy)
+    f_ref = f0
+    W = []
+    for _ in range(10):
+        f_ref, drho, _, Ex, Ey = vo.vlasov_step_2d(f_ref, 0.1, g, True, grid_arrays=arrays)
+        W.append(vo.field_energy_2d(Ex, Ey, g.dx, g.dy))
+    f_dev = run.state_f64().cpu().numpy()
+    err_f = np.max(np.abs(f_dev - f_ref)) / np.max(np.abs(f_ref))
+    err_w = np.max(np.abs(energy.cpu().numpy() - np.array(W))) / max(W)
+    assert err_f <= 2e-6, err_f
+    assert err_w <= 1e-6, err_w
+    assert np.max(np.abs(run.delta_rho.cpu().numpy() - drho)) <= 1e-6
+    assert abs(f_dev.sum() - f0.sum()) / f0.sum() <= 1e-9
+    assert run.state.dtype == torch.float32
+
+
+def test_vlasov2d_float32_driver_deviation_storage(device, tmp_path):
+    v2, runner, _ = _mods()
+    kw = dict(Nx=64, Ny=64, Nvx=32, Nvy=128, Lx=4 * math.pi, Ly=4 * math.pi, Vx_max=6.0, Vy_max=6.0, nbiter=20,
+              diag_steps=10, dt=0.1, physics_mode=True, epsilon=0.05, device=device)
+    assert runner.deviation_supported((64, 64, 32, 128))
+    a = v2.Vlasov(dtype="float64", out_dir=str(tmp_path / "a"), **kw)
+    b = v2.Vlasov(dtype="float32", fp32_storage="deviation", out_dir=str(tmp_path / "b"), **kw)
+    c = v2.Vlasov(dtype="float32", out_dir=str(tmp_path / "c"), **kw)
+    sa, sb, sc = (s.run(s.initialize()) for s in (a, b, c))
+    assert sb.variables.fn.dtype == torch.float32
+    eb = np.max(np.abs(b.energy_history - a.energy_history)) / np.max(a.energy_history)
+    ec = np.max(np.abs(c.energy_history - a.energy_history)) / np.max(a.energy_history)
+    assert eb <= 1e-6, eb
+    assert eb < ec
