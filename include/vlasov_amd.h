@@ -334,6 +334,10 @@ int vp_nn_bias_grad(const float* dy, float* db, int64_t rows, int cols, vp_strea
  * receives per-block sums of squared errors */
 int vp_nn_mse(const float* pred, const float* target, float* grad, float* partial, int64_t n,
               vp_stream_t stream);
+/* weighted squared error: partial[b] = sum over block b (256 elements) of
+ * w (pred - target)^2, grad = 2 w (pred - target) (grad may be NULL) */
+int vp_nn_wmse(const float* pred, const float* target, const float* w, float* grad, float* partial,
+               int64_t n, vp_stream_t stream);
 /* Adam with bias correction at step >= 1 (float32 master weights) */
 int vp_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                float eps, int step, vp_stream_t stream);

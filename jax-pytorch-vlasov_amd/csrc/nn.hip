@@ -199,6 +199,29 @@ __global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ pred
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Weighted squared error: partial[block] = sum of w d^2 over the block,
+// grad = 2 w d (d = pred - target).  With w = 1/n it is mse_kernel.
+__global__ __launch_bounds__(256) void wmse_kernel(const float* __restrict__ pred, const float* __restrict__ target,
+                                                   const float* __restrict__ w, float* __restrict__ grad,
+                                                   float* __restrict__ partial, int64_t n) {
+    __shared__ float red[256];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    float e2 = 0.f;
+    if (i < n) {
+        const float d = pred[i] - target[i];
+        const float wd = w[i] * d;
+        if (grad) grad[i] = 2.f * wd;
+        e2 = wd * d;
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // Adam (bias-corrected), float32 master weights.
 __global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
                             float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
@@ -283,6 +306,15 @@ extern "C" int vp_nn_mse(const float* pred, const float* target, float* grad, fl
     VP_RETURN_IF(!pred || !target || !partial || n < 1, VP_ERR_ARG);
     hipLaunchKernelGGL(vp::mse_kernel, vp::grid1d(n), dim3(256), 0, static_cast<hipStream_t>(stream), pred, target,
                        grad, partial, n);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+extern "C" int vp_nn_wmse(const float* pred, const float* target, const float* w, float* grad, float* partial,
+                          int64_t n, vp_stream_t stream) {
+    VP_RETURN_IF(!pred || !target || !w || !partial || n < 1, VP_ERR_ARG);
+    hipLaunchKernelGGL(vp::wmse_kernel, vp::grid1d(n), dim3(256), 0, static_cast<hipStream_t>(stream), pred, target,
+                       w, grad, partial, n);
     VP_CHECK_LAUNCH();
     return VP_OK;
 }

@@ -84,6 +84,7 @@ _SIGS = {
     "vp_nn_bias_add": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),
     "vp_nn_bias_grad": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),
     "vp_nn_mse": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "vp_nn_wmse": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "vp_nn_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                            ctypes.c_float, c_int, c_vp]),
     "vp_tune_set": (c_int, [c_int, c_int]),

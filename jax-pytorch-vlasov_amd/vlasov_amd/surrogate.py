@@ -201,6 +201,14 @@ class FNO:
             check(lib().vp_nn_adam(w.data_ptr(), self.g[k].data_ptr(), self.m[k].data_ptr(), self.v[k].data_ptr(),
                                    w.numel(), lr, b1, b2, eps, self.step, self._s()), "adam")
 
+    def wmse(self, y: torch.Tensor, target: torch.Tensor, w: torch.Tensor, grad: torch.Tensor | None) -> torch.Tensor:
+        """Weighted sum of squared errors (device scalar); grad <- 2 w (y - target)."""
+        n = y.numel()
+        part = self._buf("mse_part", ((n + 255) // 256,))
+        check(lib().vp_nn_wmse(y.data_ptr(), target.data_ptr(), w.data_ptr(),
+                               None if grad is None else grad.data_ptr(), part.data_ptr(), n, self._s()), "wmse")
+        return part.sum()
+
     def mse(self, y: torch.Tensor, target: torch.Tensor, grad: torch.Tensor | None) -> torch.Tensor:
         """Sum of squared errors (device scalar); grad <- 2 (y - target) / n."""
         n = y.numel()
@@ -315,14 +323,21 @@ class Surrogate:
         return torch.log(torch.maximum(W, floor))
 
     def fit(self, params: np.ndarray, W: torch.Tensor, iters: int = 16000, batch: int = 128, lr: float = 3e-3,
-            seed: int = 0, log=None) -> list:
-        """Adam on the mean squared error of the normalized log history;
-        cosine learning-rate decay.  Returns (iteration, loss) every 50
-        iterations; `log` (a callable) receives them as they come."""
+            seed: int = 0, log=None, uniform: float = 1.0) -> list:
+        """Adam on the weighted mean squared error of the normalized log
+        history; cosine learning-rate decay.  Per run the weight of time t
+        is uniform / T + (1 - uniform) W_t^2 / sum W^2: the second part
+        makes the loss the squared relative L2 error of the history (to
+        first order), the metric the solver judges the surrogate by.
+        Returns (iteration, loss) every 50 iterations; `log` (a callable)
+        receives them as they come."""
         logW = self._log_target(W)
         self.mu = float(logW.mean())
         self.sd = float(logW.std())
         Y = ((logW - self.mu) / self.sd).float().contiguous()
+        W2 = W.double() ** 2
+        Wt = (uniform / self.steps + (1.0 - uniform) * W2 / W2.sum(dim=1, keepdim=True)) / batch
+        Wt = Wt.float().contiguous()
         X = features(params, self.steps, self.device)
         n = len(params)
         gen = torch.Generator(device="cpu").manual_seed(seed)
@@ -332,13 +347,14 @@ class Surrogate:
             idx = torch.randint(0, n, (batch,), generator=gen).to(self.device)
             xb = X.index_select(0, idx).contiguous()
             yb = Y.index_select(0, idx).contiguous()
+            wb = Wt.index_select(0, idx).contiguous()
             out = self.net.forward(xb)
-            sse = self.net.mse(out, yb, dy)
+            sse = self.net.wmse(out, yb, wb, dy)
             self.net.backward(dy)
             lr_t = 1e-5 + 0.5 * (lr - 1e-5) * (1.0 + math.cos(math.pi * it / max(iters - 1, 1)))
             self.net.adam(lr_t)
             if it % 50 == 0 or it == iters - 1:
-                losses.append((it, float(sse) / out.numel()))
+                losses.append((it, float(sse)))
                 if log is not None:
                     log(losses[-1])
         return losses

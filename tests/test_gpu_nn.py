@@ -95,3 +95,30 @@ def test_elementwise_kernels(device):
     L.check(lib.vp_nn_adam(w.data_ptr(), gw.data_ptr(), m.data_ptr(), v.data_ptr(), 50, 0.01, 0.9, 0.999, 1e-8, 1, s),
             "adam")
     np.testing.assert_allclose(w.cpu().numpy(), -0.01 * np.sign(gw.cpu().numpy()), rtol=1e-4)
+
+
+def test_weighted_squared_error(device):
+    L = _lib()
+    lib = L.lib()
+    s = torch.cuda.current_stream().cuda_stream
+    rng = np.random.default_rng(4)
+    n = 1537
+    p = rng.standard_normal(n).astype(np.float32)
+    t = rng.standard_normal(n).astype(np.float32)
+    w = rng.random(n).astype(np.float32)
+    tp, tt, tw = (torch.from_numpy(a).to(device) for a in (p, t, w))
+    grad = torch.empty_like(tp)
+    part = torch.empty((n + 255) // 256, device=device)
+    L.check(lib.vp_nn_wmse(tp.data_ptr(), tt.data_ptr(), tw.data_ptr(), grad.data_ptr(), part.data_ptr(), n, s),
+            "wmse")
+    d = p.astype(np.float64) - t
+    np.testing.assert_allclose(float(part.sum()), np.sum(w * d * d), rtol=1e-5)
+    np.testing.assert_allclose(grad.cpu().numpy(), 2 * w * d, rtol=1e-6, atol=1e-7)
+    # w = 1/n is the plain mean squared error
+    tw.fill_(1.0 / n)
+    g2, part2 = torch.empty_like(tp), torch.empty_like(part)
+    L.check(lib.vp_nn_wmse(tp.data_ptr(), tt.data_ptr(), tw.data_ptr(), grad.data_ptr(), part.data_ptr(), n, s),
+            "wmse")
+    L.check(lib.vp_nn_mse(tp.data_ptr(), tt.data_ptr(), g2.data_ptr(), part2.data_ptr(), n, s), "mse")
+    np.testing.assert_allclose(grad.cpu().numpy(), g2.cpu().numpy(), rtol=1e-6)
+    np.testing.assert_allclose(float(part.sum()), float(part2.sum()), rtol=1e-5)
