@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 512-instance 1D1V ensemble")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-surrogate", action="store_true", help="skip training the Fourier-operator surrogate")
-    ap.add_argument("--surrogate-iters", type=int, default=12000)
+    ap.add_argument("--surrogate-iters", type=int, default=24000)
     ap.add_argument("--stream-only", action="store_true",
                     help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
     return ap.parse_args()
@@ -238,7 +238,10 @@ def surrogate_bench(iters: int, ntrain: int = 20000, ntest: int = 1000, steps: i
     t_solve = time.perf_counter() - t0
     sur = sg.Surrogate(steps)
     t0 = time.perf_counter()
-    losses = sur.fit(ptr, Wtr, iters=iters)
+    _progress(f"surrogate: data {t_data:.1f} s; training {iters} iterations")
+    losses = sur.fit(ptr, Wtr, iters=iters, uniform=0.1,
+                     log=lambda x: _progress(f"surrogate: iteration {x[0]} loss {x[1]:.3g}") if x[0] % 4000 == 0
+                     else None)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     sur.predict(pte[:8])
@@ -251,6 +254,7 @@ def surrogate_bench(iters: int, ntrain: int = 20000, ntest: int = 1000, steps: i
     elog = (torch.log(Wp) - torch.log(Wte)).abs().mean(dim=1).cpu().numpy()
     return {
         "model": "1D Fourier neural operator over time: 4 layers, width 64, 32 modes, BF16 MFMA / FP32 accumulate",
+        "loss": "log W_E error weighted 0.1 / T + 0.9 W_t^2 / sum W^2 per run (the relative L2 error to first order)",
         "train_runs": ntrain, "test_runs": ntest, "steps": steps, "grid": [128, 256],
         "data_seconds": t_data, "train_iters": iters, "train_seconds": t_train,
         "final_loss": losses[-1][1],
@@ -261,6 +265,11 @@ def surrogate_bench(iters: int, ntrain: int = 20000, ntest: int = 1000, steps: i
         "speedup_vs_solver": t_solve / t_pred,
         "fallback_rule": "judged relative L2 error of the history > 2% -> the request is answered by the solver",
     }
+
+
+def _progress(msg: str) -> None:
+    """Progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def _steps_within(rel_err, tol: float) -> int:
@@ -415,12 +424,16 @@ def main():
     del run
     torch.cuda.empty_cache()
     if not args.no_fp32 and runner.fused_supported(f0.shape, torch.float32):
+        _progress("fp32 storage")
         out["fp32_storage"] = fp32_storage(grid, f0, args.steps, args.warmup, args.fp32_error_steps)
     if not args.no_ensemble:
+        _progress("ensemble")
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
     if not args.no_surrogate:
+        _progress("surrogate")
         out["surrogate"] = surrogate_bench(args.surrogate_iters)
     if not args.no_cpu_baseline:
+        _progress("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)
 

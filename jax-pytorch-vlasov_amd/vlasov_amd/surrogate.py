@@ -323,7 +323,7 @@ class Surrogate:
         return torch.log(torch.maximum(W, floor))
 
     def fit(self, params: np.ndarray, W: torch.Tensor, iters: int = 16000, batch: int = 128, lr: float = 3e-3,
-            seed: int = 0, log=None, uniform: float = 1.0) -> list:
+            seed: int = 0, log=None, uniform: float = 0.1) -> list:
         """Adam on the weighted mean squared error of the normalized log
         history; cosine learning-rate decay.  Per run the weight of time t
         is uniform / T + (1 - uniform) W_t^2 / sum W^2: the second part

@@ -114,11 +114,11 @@ def test_weighted_squared_error(device):
     d = p.astype(np.float64) - t
     np.testing.assert_allclose(float(part.sum()), np.sum(w * d * d), rtol=1e-5)
     np.testing.assert_allclose(grad.cpu().numpy(), 2 * w * d, rtol=1e-6, atol=1e-7)
-    # w = 1/n is the plain mean squared error
+    # w = 1/n: the gradient of vp_nn_mse, whose partial sums are n times these
     tw.fill_(1.0 / n)
     g2, part2 = torch.empty_like(tp), torch.empty_like(part)
     L.check(lib.vp_nn_wmse(tp.data_ptr(), tt.data_ptr(), tw.data_ptr(), grad.data_ptr(), part.data_ptr(), n, s),
             "wmse")
     L.check(lib.vp_nn_mse(tp.data_ptr(), tt.data_ptr(), g2.data_ptr(), part2.data_ptr(), n, s), "mse")
     np.testing.assert_allclose(grad.cpu().numpy(), g2.cpu().numpy(), rtol=1e-6)
-    np.testing.assert_allclose(float(part.sum()), float(part2.sum()), rtol=1e-5)
+    np.testing.assert_allclose(float(part.sum()) * n, float(part2.sum()), rtol=1e-5)
