@@ -418,7 +418,7 @@ class Vlasov:
     def __init__(self, Nx: int = 128, Nv: int = 128, L: float = 2.0 * math.pi, V_max: float = 6.0,
                  nbiter: int = 10, diag_steps: int = 10, dt: float = 0.05, out_dir: str = "data",
                  physics_mode: bool = False, epsilon: float = 0.001, dtype: str = "float64",
-                 solver: int = 0, device=None) -> None:
+                 solver: int = 0, device=None, snapshot_format: str = "npz") -> None:
         """Same parameters as the reference constructor (P1:770-876).
 
         ``solver`` selects the step as in the reference: 0 vlasov_step,
@@ -436,6 +436,9 @@ class Vlasov:
         self.diag_steps = diag_steps
         self.dt = dt
         self.out_dir = out_dir
+        if snapshot_format not in ("npz", "bin"):
+            raise ValueError(f"snapshot_format must be 'npz' or 'bin', got {snapshot_format!r}")
+        self.snapshot_format = snapshot_format
         self.physics_mode = physics_mode
         self.epsilon = epsilon
         solver_fns = {0: vlasov_step, 1: vlasov_step_matrix_diag, 2: vlasov_step_gather_free}
@@ -493,15 +496,34 @@ class Vlasov:
 
     def save_diag(self, state: SimulationState) -> None:
         """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray),
-        with the moments (mass, kinetic energy, L2) of f."""
+        or with snapshot_format='bin' to the Kokkos solver's raw
+        ``fn/f0/rho/phi_%06d.bin`` files; records the moments (mass, kinetic
+        energy, L2) of f."""
         mom = moments(state.variables.fn, self.grid).cpu().numpy()
         self.moment_history.append((int(state.step), float(state.t), *[float(m) for m in mom]))
+        if self.snapshot_format == "bin":
+            from . import snapshots
+            snapshots.write_snapshot(self.out_dir, state.step, state.variables.fn, state.variables.f0,
+                                     state.field.rho, state.field.phi)
+            return
         filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
         np.savez(filename, f=state.variables.fn.detach().cpu().numpy(),
                  f0=state.variables.f0.detach().cpu().numpy(),
                  rho=state.field.rho.detach().cpu().numpy(), phi=state.field.phi.detach().cpu().numpy(),
                  time=np.array([float(state.t)]), x=self.grid.x.cpu().numpy(), v=self.grid.v.cpu().numpy(),
                  step=int(state.step), dt=self.dt, physics_mode=int(self.physics_mode), moments=mom)
+
+    def restart(self, step: int) -> SimulationState:
+        """State of the raw binary snapshot of iteration ``step`` in out_dir
+        (written with snapshot_format='bin', or by the reference's Kokkos
+        solver); run() continues from it."""
+        from . import snapshots
+        g = self.grid
+        a = snapshots.read_snapshot(self.out_dir, step, {"fn": (g.Nx, g.Nv), "f0": (g.Nv,), "rho": (g.Nx,),
+                                                         "phi": (g.Nx,)})
+        t = {k: torch.from_numpy(v.astype(np.float64)).to(self.device) for k, v in a.items()}
+        return SimulationState(field=Field(t["rho"], t["phi"]), variables=Variables(fn=t["fn"], f0=t["f0"]),
+                               t=float(step) * self.dt, step=int(step))
 
     def run(self, state: SimulationState) -> SimulationState:
         """Time loop of P1:943-994.  ``self.energy_history`` receives the field

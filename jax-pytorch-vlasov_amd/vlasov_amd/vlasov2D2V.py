@@ -407,7 +407,7 @@ class Vlasov:
     def __init__(self, Nx=32, Ny=32, Nvx=32, Nvy=32, Lx=2.0 * math.pi, Ly=2.0 * math.pi, Vx_max=6.0,
                  Vy_max=6.0, nbiter=10, diag_steps=10, dt=0.05, out_dir="data", physics_mode=False,
                  epsilon=0.001, dtype: str = "float64", solver: int = 0, device=None,
-                 fp32_storage: str = "plain"):
+                 fp32_storage: str = "plain", snapshot_format: str = "npz"):
         """Same parameters as the reference constructor (P2:1236-1314).
 
         ``solver`` selects the step as the reference's ``solver_fns`` does:
@@ -436,6 +436,9 @@ class Vlasov:
         self.diag_steps = diag_steps
         self.dt = dt
         self.out_dir = out_dir
+        if snapshot_format not in ("npz", "bin"):
+            raise ValueError(f"snapshot_format must be 'npz' or 'bin', got {snapshot_format!r}")
+        self.snapshot_format = snapshot_format
         self.physics_mode = physics_mode
         self.epsilon = epsilon
         solver_fns = {0: vlasov_step_impl, 1: vlasov_step_matrix_diag, 2: vlasov_step_gather_free}
@@ -509,10 +512,17 @@ class Vlasov:
 
     def save_diag(self, state: SimulationState) -> None:
         """Snapshot to ``field_%06d.npz`` (the reference writes netCDF via xarray),
-        with the moments (mass, kinetic energy, L2) of f."""
+        or with snapshot_format='bin' to the Kokkos solver's raw
+        ``fn/f0/rho/phi_%06d.bin`` files; records the moments (mass, kinetic
+        energy, L2) of f."""
         g = self.grid
         mom = moments(state.variables.fn, g).cpu().numpy()
         self.moment_history.append((int(state.step), float(state.t), *[float(m) for m in mom]))
+        if self.snapshot_format == "bin":
+            from . import snapshots
+            snapshots.write_snapshot(self.out_dir, state.step, state.variables.fn, state.variables.f0,
+                                     state.field.rho, state.field.phi)
+            return
         filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
         np.savez(filename, f=state.variables.fn.detach().cpu().numpy(), f0=state.variables.f0.detach().cpu().numpy(),
                  rho=state.field.rho.detach().cpu().numpy(), phi=state.field.phi.detach().cpu().numpy(),
@@ -520,6 +530,20 @@ class Vlasov:
                  vx=g.vx.cpu().numpy(), vy=g.vy.cpu().numpy(), step=int(state.step), dt=self.dt,
                  Lx=g.Lx, Ly=g.Ly, Vx_max=g.Vx_max, Vy_max=g.Vy_max, physics_mode=int(self.physics_mode),
                  moments=mom)
+
+    def restart(self, step: int) -> SimulationState:
+        """State of the raw binary snapshot of iteration ``step`` in out_dir
+        (written with snapshot_format='bin', or by the reference's Kokkos
+        solver); run() continues from it.  f keeps the file's precision
+        (float32 files give the FP32-storage variant's f)."""
+        from . import snapshots
+        g = self.grid
+        a = snapshots.read_snapshot(self.out_dir, step, {"fn": (g.Nx, g.Ny, g.Nvx, g.Nvy), "f0": (g.Nvx, g.Nvy),
+                                                         "rho": (g.Nx, g.Ny), "phi": (g.Nx, g.Ny)})
+        fn = torch.from_numpy(a["fn"]).to(self.device).to(self.f_dtype)
+        t = {k: torch.from_numpy(a[k].astype(np.float64)).to(self.device) for k in ("f0", "rho", "phi")}
+        return SimulationState(field=Field(t["rho"], t["phi"]), variables=Variables(fn=fn, f0=t["f0"]),
+                               t=float(step) * self.dt, step=int(step))
 
     def run(self, state: SimulationState) -> SimulationState:
         """Time loop of P2:1392-1435.

@@ -107,3 +107,36 @@ def test_command_line_log_matches_reference_parsers(device, tmp_path, monkeypatc
     assert float(_ELAPSED.search(text).group(1)) > 0
     assert _GRID.search(text).group(1).replace(" ", "") == grid
     assert int(_STEPS.search(text).group(1)) == (4 if two_d else 6)
+
+
+def test_binary_snapshots_and_restart_2d(device, tmp_path):
+    """A 2D run that writes the Kokkos solver's raw snapshots, restarted from
+    the middle one, ends bit for bit where the uninterrupted run ends."""
+    from vlasov_amd import snapshots
+    from vlasov_amd import vlasov2D2V as v2
+    kw = dict(Nx=32, Ny=32, Nvx=32, Nvy=32, Lx=4 * math.pi, Ly=4 * math.pi, Vx_max=6.0, Vy_max=6.0,
+              diag_steps=3, dt=0.1, physics_mode=True, epsilon=0.05, device=device, snapshot_format="bin")
+    a = v2.Vlasov(nbiter=6, out_dir=str(tmp_path), **kw)
+    sa = a.run(a.initialize())
+    assert snapshots.snapshot_steps(tmp_path) == [0, 3, 6]
+    b = v2.Vlasov(nbiter=3, out_dir=str(tmp_path), **kw)
+    st = b.restart(3)
+    assert st.step == 3 and st.variables.fn.shape == (32, 32, 32, 32)
+    sb = b.run(st)
+    assert torch.equal(sb.variables.fn, sa.variables.fn)
+    np.testing.assert_array_equal(b.energy_history, a.energy_history[3:])
+    last = snapshots.read_snapshot(tmp_path, 6, {"fn": (32, 32, 32, 32), "f0": (32, 32), "rho": (32, 32),
+                                                "phi": (32, 32)})
+    np.testing.assert_array_equal(last["fn"], sa.variables.fn.cpu().numpy())
+
+
+def test_binary_snapshots_and_restart_1d(device, tmp_path):
+    from vlasov_amd import vlasov1D1V as v1
+    kw = dict(Nx=64, Nv=96, L=4 * math.pi, V_max=6.0, diag_steps=5, dt=0.1, physics_mode=True, epsilon=0.01,
+              device=device, snapshot_format="bin")
+    a = v1.Vlasov(nbiter=10, out_dir=str(tmp_path), **kw)
+    sa = a.run(a.initialize())
+    b = v1.Vlasov(nbiter=5, out_dir=str(tmp_path), **kw)
+    sb = b.run(b.restart(5))
+    assert torch.equal(sb.variables.fn, sa.variables.fn)
+    np.testing.assert_array_equal(b.energy_history, a.energy_history[5:])
