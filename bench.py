@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 512-instance 1D1V ensemble")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-surrogate", action="store_true", help="skip training the Fourier-operator surrogate")
+    ap.add_argument("--no-reference-configs", action="store_true",
+                    help="skip the per-step times on the configurations the reference published")
     ap.add_argument("--surrogate-iters", type=int, default=24000)
     ap.add_argument("--stream-only", action="store_true",
                     help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
@@ -267,6 +269,80 @@ def surrogate_bench(iters: int, ntrain: int = 20000, ntest: int = 1000, steps: i
     }
 
 
+# The reference's published per-step times for the configurations below
+# (elapsed time / steps from the cited logs; Landau damping, dt 0.05, L 4 pi,
+# V_max 5, eps 1e-3).
+REFERENCE_TIMES = {
+    "2d2v_128_fp64": [
+        ("Kokkos s1", "H200", 17.16, "artifact/vlasov2d_2v/kokkos/d875542/vlp2d_2v_H200_fp64_Nx128_Nvx128_solver1.txt:29"),
+        ("PyTorch s2", "H200", 17.23, "artifact/vlasov2d_2v/pytorch/d875542/vlp2d_2v_H200_float64_Nx128_Nvx128_solver2.txt:6"),
+        ("Kokkos s1", "MI250X GCD", 48.15,
+         "artifact/vlasov2d_2v/kokkos/d1904d7/vlp2d_2v_MI250X_fp64_Nx128_Nvx128_solver1.txt:96")],
+    "2d2v_128_fp32": [
+        ("PyTorch s2", "H200", 9.49, "artifact/vlasov2d_2v/pytorch/d875542/vlp2d_2v_H200_float32_Nx128_Nvx128_solver2.txt:6"),
+        ("Kokkos s1", "MI250X GCD", 39.93,
+         "artifact/vlasov2d_2v/kokkos/d1904d7/vlp2d_2v_MI250X_fp32_Nx128_Nvx128_solver1.txt:96")],
+    "2d2v_64_fp64": [
+        ("PyTorch s2", "H200", 0.841, "artifact/vlasov2d_2v/pytorch/d875542/vlp2d_2v_H200_float64_Nx64_Nvx64_solver2.txt:6"),
+        ("Kokkos s1", "MI250X GCD", 3.20, "artifact/vlasov2d_2v/kokkos/d1904d7/vlp2d_2v_MI250X_fp64_Nx64_Nvx64_solver1.txt:96")],
+    "1d1v_128_fp64": [
+        ("PyTorch s2", "H200", 0.363, "artifact/vlasov1d_1v/pytorch/d875542/vlp1d_1v_H200_float64_N128_solver2.txt:6"),
+        ("Kokkos s1", "MI250X GCD", 0.135, "artifact/vlasov1d_1v/kokkos/d1904d7/vlp1d_1v_MI250X_fp64_N128_solver1.txt:96")],
+    "1d1v_4096_fp64": [
+        ("PyTorch s2", "H200", 0.518, "artifact/vlasov1d_1v/pytorch/d875542/vlp1d_1v_H200_float64_N4096_solver2.txt:6"),
+        ("Kokkos s1", "MI250X GCD", 1.379,
+         "artifact/vlasov1d_1v/kokkos/d1904d7/vlp1d_1v_MI250X_fp64_N4096_solver1.txt:96")],
+}
+
+
+def reference_configs(ms_2d_fp64: float, ms_2d_fp32) -> dict:
+    """This library's per-step time on the configurations the reference
+    published, next to those published times (other hardware).  2D 128^4
+    comes from the main and FP32 legs; 2D 64^4 runs the fused runner, 1D1V
+    the drop-in step (vlasov1D1V.step_with_field) from Python, one call per
+    step, as the reference's own loop does."""
+    import torch
+
+    from vlasov_amd import configs, runner
+    from vlasov_amd import vlasov1D1V as v1
+    from vlasov_amd import vlasov2D2V as v2
+
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(steps)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    ours = {"2d2v_128_fp64": ms_2d_fp64, "2d2v_128_fp32": ms_2d_fp32}
+    g = v2.Grid.create(64, 64, 64, 64, 4 * math.pi, 4 * math.pi, 5.0, 5.0, device="cuda")
+    run = runner.Run2D(g, configs.landau_2d(g, eps=1e-3), 0.05, True)
+    run.advance(5)
+    ours["2d2v_64_fp64"] = timed(run.advance, 128)
+    del run
+    for n, steps in ((128, 1000), (4096, 50)):
+        g1 = v1.Grid.create(n, n, 4 * math.pi, 5.0, device="cuda")
+        state = {"f": configs.landau_1d(g1, eps=1e-3, k=0.5)}
+
+        def steps_1d(k):
+            f = state["f"]
+            for _ in range(k):
+                f, _, _, _ = v1.step_with_field(f, 0.05, g1.v, g1.inv_kx, g1.inv_kx_sq, g1.dx, g1.dv, True)
+            state["f"] = f
+
+        steps_1d(5)
+        ours[f"1d1v_{n}_fp64"] = timed(steps_1d, steps)
+    out = {}
+    for key, refs in REFERENCE_TIMES.items():
+        if ours.get(key) is None:
+            continue
+        best = min(r[2] for r in refs)
+        out[key] = {"ms_per_step": ours[key],
+                    "reference": [{"impl": a, "hardware": b, "ms_per_step": c, "source": d} for a, b, c, d in refs],
+                    "speedup_vs_fastest_reference": best / ours[key]}
+    return out
+
+
 def _progress(msg: str) -> None:
     """Progress line on stderr (stdout carries only the result line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -405,7 +481,10 @@ def main():
         "ms_per_step": t_step * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        # fastest published 2D2V 128^4 FP64 rate: Kokkos s1 on H200, 17.16 ms/step
+        "vs_baseline": (cells / t_step) / (cells / 17.16e-3) if n == 128 else None,
+        "vs_baseline_ref": "2D2V 128^4 FP64 cell-updates/s of the reference, Kokkos s1 on H200 (17.16 ms/step, "
+                           "artifact/vlasov2d_2v/kokkos/d875542/vlp2d_2v_H200_fp64_Nx128_Nvx128_solver1.txt:29)",
         "dtype": "f64",
         "data": "synthetic (closed-form two-stream initial condition)",
         "config": {"workload": f"2D2V two-stream instability {n}^4 FP64, k=0.2, v in [-8,8], dt=0.05",
@@ -429,6 +508,10 @@ def main():
     if not args.no_ensemble:
         _progress("ensemble")
         out["ensemble"] = ensemble_bench(args.ensemble_steps)
+    if not args.no_reference_configs:
+        _progress("reference configurations")
+        fp32_ms = out.get("fp32_storage", {}).get("ms_per_step")
+        out["reference_configs"] = reference_configs(t_step * 1e3, fp32_ms)
     if not args.no_surrogate:
         _progress("surrogate")
         out["surrogate"] = surrogate_bench(args.surrogate_iters)

@@ -37,6 +37,95 @@ __global__ __launch_bounds__(1024) void poisson1d_kernel(const double* __restric
     poisson1d_block(rho, nx, inv_kx, inv_kx_sq, E, phi, ws, ws + nx, ws + nx + (nx / 2 + 1));
 }
 
+// 1D, many workgroups (nx from 512 to 4096): the DFT sums of
+// poisson1d_block split over 16 lanes each (16 outputs per workgroup of 256
+// threads, each lane one sixteenth of the sum, merged by a lane-shuffle
+// tree), with the twiddles, rho and the half spectra staged in LDS.  Same
+// terms, another summation order: agrees with the one-workgroup solve to
+// rounding.  ws: Ek, Pk after the twiddle slot of the one-workgroup layout.
+constexpr int kP1dParts = 16;
+
+__device__ __forceinline__ double p1d_merge(double v) {
+#pragma unroll
+    for (int off = kP1dParts / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kP1dParts);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void p1d_fwd_kernel(const double* __restrict__ rho, int nx,
+                                                      const double* __restrict__ inv_kx,
+                                                      const double* __restrict__ inv_kx_sq, double2* __restrict__ Ek,
+                                                      double2* __restrict__ Pk) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* tw = reinterpret_cast<double2*>(smem_raw);
+    double* r = reinterpret_cast<double*>(tw + nx);
+    fill_twiddles(tw, nx);
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) r[x] = rho[x];
+    __syncthreads();
+    const int nk = nx / 2 + 1;
+    const int part = threadIdx.x % kP1dParts;
+    const int k = blockIdx.x * (256 / kP1dParts) + threadIdx.x / kP1dParts;
+    const int kk = k < nk ? k : 0;  // lanes past nk compute a dummy sum (every lane joins the shuffles)
+    const int x0 = static_cast<int>(static_cast<int64_t>(part) * nx / kP1dParts);
+    const int x1 = static_cast<int>(static_cast<int64_t>(part + 1) * nx / kP1dParts);
+    double re = 0.0, im = 0.0;
+    int ti = static_cast<int>((static_cast<int64_t>(kk) * x0) % nx);  // (k * x) mod nx, stepped
+    for (int x = x0; x < x1; ++x) {
+        const double2 t = tw[ti];
+        re += r[x] * t.x;
+        im -= r[x] * t.y;
+        ti += kk;
+        ti -= (ti >= nx) ? nx : 0;
+    }
+    re = p1d_merge(re);
+    im = p1d_merge(im);
+    if (part == 0 && k < nk) {
+        Ek[k] = make_double2(im * inv_kx[k], -re * inv_kx[k]);
+        Pk[k] = make_double2(re * inv_kx_sq[k], im * inv_kx_sq[k]);
+    }
+}
+
+__global__ __launch_bounds__(256) void p1d_inv_kernel(const double2* __restrict__ Ek_g,
+                                                      const double2* __restrict__ Pk_g, int nx,
+                                                      double* __restrict__ E, double* __restrict__ phi) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int nk = nx / 2 + 1;
+    double2* tw = reinterpret_cast<double2*>(smem_raw);
+    double2* Ek = tw + nx;
+    double2* Pk = Ek + nk;
+    fill_twiddles(tw, nx);
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+        Ek[k] = Ek_g[k];
+        Pk[k] = Pk_g[k];
+    }
+    __syncthreads();
+    const int part = threadIdx.x % kP1dParts;
+    const int x = blockIdx.x * (256 / kP1dParts) + threadIdx.x / kP1dParts;
+    const int xx = x < nx ? x : 0;
+    const int khalf = (nx + 1) / 2;  // bins 1..khalf-1 are doubled
+    const int k0 = 1 + static_cast<int>(static_cast<int64_t>(part) * (khalf - 1) / kP1dParts);
+    const int k1 = 1 + static_cast<int>(static_cast<int64_t>(part + 1) * (khalf - 1) / kP1dParts);
+    double se = 0.0, sp = 0.0;
+    int ti = static_cast<int>((static_cast<int64_t>(xx) * k0) % nx);  // (x * k) mod nx, stepped
+    for (int k = k0; k < k1; ++k) {
+        const double2 t = tw[ti];
+        se += 2.0 * (Ek[k].x * t.x - Ek[k].y * t.y);
+        sp += 2.0 * (Pk[k].x * t.x - Pk[k].y * t.y);
+        ti += xx;
+        ti -= (ti >= nx) ? nx : 0;
+    }
+    se = Ek[0].x + p1d_merge(se);
+    sp = Pk[0].x + p1d_merge(sp);
+    if (part != 0 || x >= nx) return;
+    if ((nx & 1) == 0) {
+        const double sgn = (x & 1) ? -1.0 : 1.0;
+        se += Ek[nx / 2].x * sgn;
+        sp += Pk[nx / 2].x * sgn;
+    }
+    const double inv_n = 1.0 / static_cast<double>(nx);
+    E[x] = se * inv_n;
+    phi[x] = sp * inv_n;
+}
+
 // ---------------------------------------------------------------------------
 // 2D: compute_poisson_real (vlasov2D2V.py:488-517).
 // ws: R[nx][nky], T[3][nx][nky]  (double2), nky = ny/2 + 1
@@ -200,6 +289,31 @@ int poisson1d(const double* rho, int64_t nx, const double* inv_kx, const double*
     VP_RETURN_IF(!rho || !inv_kx || !inv_kx_sq || !E || !phi || !ws, VP_ERR_ARG);
     VP_RETURN_IF(nx < 1 || nx > 8192, VP_ERR_SHAPE);
     VP_RETURN_IF(ws_bytes < poisson1d_ws(nx), VP_ERR_WORKSPACE);
+    if (nx >= 512 && nx <= 4096) {
+        // many workgroups; LDS: fwd tw + rho (96 KB at 4096), inv tw + Ek + Pk (128 KB)
+        const int n = static_cast<int>(nx), nk = n / 2 + 1;
+        const size_t lds_f = static_cast<size_t>(n) * (sizeof(double2) + sizeof(double));
+        const size_t lds_i = static_cast<size_t>(n + 2 * nk) * sizeof(double2);
+        static bool attr = false;
+        if (!attr) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&p1d_fwd_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 4096 * 24);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(reinterpret_cast<const void*>(&p1d_inv_kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (4096 + 2 * 2049) * 16);
+            if (e != hipSuccess) return static_cast<int>(e);
+            attr = true;
+        }
+        double2* Ek = static_cast<double2*>(ws) + n;
+        double2* Pk = Ek + nk;
+        constexpr int per = 256 / kP1dParts;  // outputs per workgroup
+        hipLaunchKernelGGL(p1d_fwd_kernel, dim3((nk + per - 1) / per), dim3(256), lds_f, s, rho, n, inv_kx, inv_kx_sq,
+                           Ek, Pk);
+        VP_CHECK_LAUNCH();
+        hipLaunchKernelGGL(p1d_inv_kernel, dim3((n + per - 1) / per), dim3(256), lds_i, s, Ek, Pk, n, E, phi);
+        VP_CHECK_LAUNCH();
+        return VP_OK;
+    }
     hipLaunchKernelGGL(poisson1d_kernel, dim3(1), dim3(1024), 0, s, rho, static_cast<int>(nx), inv_kx,
                        inv_kx_sq, E, phi, static_cast<double2*>(ws));
     VP_CHECK_LAUNCH();

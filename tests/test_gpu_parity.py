@@ -92,12 +92,14 @@ def test_advect_dirichlet_shift_out_is_zero(device):
     assert torch.count_nonzero(out).item() == 0
 
 
-def test_rho_and_poisson_1d(device):
+@pytest.mark.parametrize("nx", [128, 512, 1000, 4096, 4098])
+def test_rho_and_poisson_1d(device, nx):
+    """One-workgroup solve (nx < 512, > 4096) and many-workgroup solve."""
     v1 = _v1()
-    g = vo.grid_1d(128, 96, 4 * math.pi, 6.0)
+    g = vo.grid_1d(nx, 96, 4 * math.pi, 6.0)
     rng = np.random.default_rng(4)
     f = (1 + 0.1 * np.cos(0.5 * g.x) + 0.05 * np.sin(1.5 * g.x))[:, None] * np.exp(-0.5 * g.v ** 2)[None, :] \
-        / math.sqrt(2 * math.pi) + 1e-3 * rng.random((128, 96))
+        / math.sqrt(2 * math.pi) + 1e-3 * rng.random((nx, 96))
     E_ref, rho_ref, phi_ref = vo.solve_poisson_real_1d(f, g.inv_kx, g.inv_kx_sq, g.dv)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
     E, rho, phi = v1.solve_poisson_real(t(f), t(g.inv_kx), t(g.inv_kx_sq), g.dv)
