@@ -44,6 +44,83 @@ __device__ __forceinline__ void weights_as(double alpha, T w[6]) {
     round_weights<T>(wd, w);
 }
 
+// Binade weight tables of the departure form.  For a line of displacement
+// disp = (vel dt)/dx and an integer index r, t = r - disp (rounded) gives
+// il = floor(t) and alpha = t - il.  When t lies in a binade [2^b, 2^(b+1))
+// with b >= 0, r is an even multiple of that binade's spacing u, so the
+// rounding of r - disp is r plus -disp rounded to the grid uZ (ties to an
+// even multiple, unaffected by r): alpha = frac(that) depends on (b, disp)
+// only.  One weight set per binade (from any r whose t falls there) is then
+// bit-identical to computing the weights element by element, without the
+// divisions.  t < 1 (the first rows) keeps the element-wise weights.
+constexpr int kBinades = 16;  // t in [1, 2^16)
+
+template <typename T>
+__device__ __forceinline__ void binade_weights(double disp, int b, T w[6]) {
+    const double lo = ldexp(1.0, b);
+    const double hi = 2.0 * lo;
+    const double c0 = ceil(lo + disp);
+    for (int d = -1; d <= 1; ++d) {
+        const double t = (c0 + d) - disp;  // the kernel's (double)r - disp for r = c0 + d
+        if (t >= lo && t < hi) {
+            weights_as<T>(t - floor(t), w);
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = T(0);  // no index of this line reaches binade b
+}
+
+// Fill tab[line][b][6] for `lines` lines (velocities vel_of(line)) with the
+// whole workgroup.
+template <typename T, typename VelOf>
+__device__ __forceinline__ void fill_binade_tables(T* tab, int lines, VelOf vel_of, double dt, double dx) {
+    for (int e = threadIdx.x; e < lines * kBinades; e += blockDim.x) {
+        const int line = e / kBinades, b = e % kBinades;
+        const double disp = (vel_of(line) * dt) / dx;
+        T w[6];
+        binade_weights<T>(disp, b, w);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tab[e * 6 + k] = w[k];
+    }
+}
+
+template <typename T, bool PERIODIC>
+__device__ __forceinline__ T interp_w(const T* tile, int64_t il, const T w[6], int n, int row_stride, int col) {
+    T v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int64_t j = il + (k - 2);
+        if (PERIODIC) {
+            v[k] = tile[wrap_index(j, n) * row_stride + col];
+        } else {
+            const int jc = static_cast<int>(j < 0 ? 0 : (j > n - 1 ? n - 1 : j));
+            const T valid = (j >= 0 && j < n) ? T(1) : T(0);
+            v[k] = tile[jc * row_stride + col] * valid;
+        }
+    }
+    return tap_sum<T>(w, v);
+}
+
+// Departure-form value of index r from a line's binade table (tab[b][6]).
+template <typename T, bool PERIODIC>
+__device__ __forceinline__ T departure_value_tab(const T* tile, int r, double disp, const T* tab, int n,
+                                                 int row_stride, int col) {
+    const double dep = static_cast<double>(r) - disp;
+    const double fl = floor(dep);
+    T w[6];
+    if (dep >= 1.0 && dep < static_cast<double>(1 << kBinades)) {
+        int e;
+        frexp(dep, &e);  // dep = m 2^e, m in [0.5, 1): binade e - 1
+        const T* tw = tab + (e - 1) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = tw[k];
+    } else {
+        weights_as<T>(dep - fl, w);
+    }
+    return interp_w<T, PERIODIC>(tile, static_cast<int64_t>(fl), w, n, row_stride, col);
+}
+
 template <typename T, bool PERIODIC>
 __device__ __forceinline__ T interp_from_lds(const T* tile, int64_t il, double alpha, int n,
                                              int row_stride, int col) {
@@ -122,7 +199,9 @@ __device__ __forceinline__ T sweep_value(const T* tile, int r, int64_t poff, con
 }
 
 // Tile = n rows (the swept axis) x C columns (contiguous inner elements).
-template <typename T, bool PERIODIC, int FORM>
+// TAB (departure form, velocity constant along each line): binade weight
+// tables of the C lines after the tile in LDS.
+template <typename T, bool PERIODIC, int FORM, bool TAB>
 __global__ __launch_bounds__(256) void advect_strided_kernel(
     const T* __restrict__ fin, T* __restrict__ fout, int n, int64_t inner, int C,
     int64_t n_chunks, SweepParams P, VelMap vm) {
@@ -136,17 +215,33 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
     const int rstep = blockDim.x / C;
     const bool cval = (i0 + c) < inner;
     const int64_t base = o * static_cast<int64_t>(n) * inner + i0 + c;
+    const int64_t ooff = split_offset(o, vm.od, vm.os, vm.no);
+    T* tab = tile + static_cast<int64_t>(n) * C;
+    if (TAB) {
+        const int64_t lim = inner - i0;
+        fill_binade_tables<T>(tab, C, [&](int line) {
+            return line < lim ? P.vel[ooff + split_offset(i0 + line, vm.id, vm.is, vm.ni)] : 0.0;
+        }, P.dt, P.dx);
+    }
     if (cval)
         for (int r = r0; r < n; r += rstep) tile[r * C + c] = fin[base + r * inner];
     __syncthreads();
     if (!cval) return;
-    const int64_t voff = split_offset(o, vm.od, vm.os, vm.no) + split_offset(i0 + c, vm.id, vm.is, vm.ni);
+    const int64_t voff = ooff + split_offset(i0 + c, vm.id, vm.is, vm.ni);
+    if (TAB) {
+        const double disp = (P.vel[voff] * P.dt) / P.dx;
+        const T* mytab = tab + c * (kBinades * 6);
+        for (int r = r0; r < n; r += rstep)
+            fout[base + r * inner] = departure_value_tab<T, PERIODIC>(tile, r, disp, mytab, n, C, c);
+        return;
+    }
     for (int r = r0; r < n; r += rstep)
         fout[base + r * inner] = sweep_value<T, PERIODIC, FORM>(tile, r, voff + r * vm.s_line, P, n, C, c);
 }
 
 // Tile = L whole contiguous lines of length n; wave w handles lines w, w+4, ...
-template <typename T, bool PERIODIC, int FORM>
+// TAB: binade weight tables of the L lines after the tile in LDS.
+template <typename T, bool PERIODIC, int FORM, bool TAB>
 __global__ __launch_bounds__(256) void advect_contig_kernel(
     const T* __restrict__ fin, T* __restrict__ fout, int64_t n_lines, int n, int L, SweepParams P, VelMap vm) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -155,6 +250,11 @@ __global__ __launch_bounds__(256) void advect_contig_kernel(
     const int64_t nl = (n_lines - l0) < L ? (n_lines - l0) : L;
     const int64_t cnt = nl * n;
     const T* src = fin + l0 * n;
+    T* tab = tile + static_cast<int64_t>(L) * n;
+    if (TAB)
+        fill_binade_tables<T>(tab, static_cast<int>(nl), [&](int line) {
+            return P.vel[split_offset(l0 + line, vm.od, vm.os, vm.no)];
+        }, P.dt, P.dx);
     for (int64_t idx = threadIdx.x; idx < cnt; idx += blockDim.x) tile[idx] = src[idx];
     __syncthreads();
     const int lane = threadIdx.x % kWave;
@@ -164,6 +264,12 @@ __global__ __launch_bounds__(256) void advect_contig_kernel(
         const int64_t voff = split_offset(l0 + l, vm.od, vm.os, vm.no);
         const T* line = tile + static_cast<int64_t>(l) * n;
         T* dst = fout + (l0 + l) * n;
+        if (TAB) {
+            const double disp = (P.vel[voff] * P.dt) / P.dx;
+            const T* ltab = tab + l * (kBinades * 6);
+            for (int j = lane; j < n; j += kWave) dst[j] = departure_value_tab<T, PERIODIC>(line, j, disp, ltab, n, 1, 0);
+            continue;
+        }
         for (int j = lane; j < n; j += kWave)
             dst[j] = sweep_value<T, PERIODIC, FORM>(line, j, voff + j * vm.s_line, P, n, 1, 0);
     }
@@ -194,33 +300,64 @@ static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis,
     }
     vm.s_line = vs[axis];
     const int block = 256;
+    // binade tables: departure form with the velocity constant along each line
+    const bool tab = FORM == kDeparture && vm.s_line == 0 && n < (1 << kBinades);
+    const size_t tab_line = tab ? static_cast<size_t>(kBinades) * 6 * sizeof(T) : 0;
+    const int64_t kMaxLds = 96 * 1024;
+    const int64_t cus = 256;  // spread small problems over at least ~2 workgroups per CU
     if (inner > 1) {
         int C = 64;
         while (C > 1 && C / 2 >= inner) C /= 2;
-        while (C > 1 && static_cast<int64_t>(n) * C * sizeof(T) > 65536) C /= 2;
-        VP_RETURN_IF(static_cast<int64_t>(n) * C * sizeof(T) > 65536, VP_ERR_UNSUPPORTED);
+        while (C > 1 && static_cast<int64_t>(n) * C * sizeof(T) + C * tab_line > kMaxLds) C /= 2;
+        while (C > 8 && outer * ((inner + C - 1) / C) < 2 * cus) C /= 2;
+        VP_RETURN_IF(static_cast<int64_t>(n) * C * sizeof(T) + C * tab_line > kMaxLds, VP_ERR_UNSUPPORTED);
         const int64_t chunks = (inner + C - 1) / C;
         const int64_t blocks = outer * chunks;
         VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
-        const size_t lds = static_cast<size_t>(n) * C * sizeof(T);
-        if (periodic)
-            hipLaunchKernelGGL((advect_strided_kernel<T, true, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
-                               n, inner, C, chunks, P, vm);
-        else
-            hipLaunchKernelGGL((advect_strided_kernel<T, false, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
-                               n, inner, C, chunks, P, vm);
+        const size_t lds = static_cast<size_t>(n) * C * sizeof(T) + C * tab_line;
+#define VP_S(PER, TB)                                                                                               \
+    {                                                                                                               \
+        static bool attr = false;                                                                                   \
+        if (!attr) {                                                                                                \
+            VP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&advect_strided_kernel<T, PER, FORM, TB>), \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds))); \
+            attr = true;                                                                                            \
+        }                                                                                                           \
+        hipLaunchKernelGGL((advect_strided_kernel<T, PER, FORM, TB>), dim3(blocks), dim3(block), lds, s, fin, fout, \
+                           n, inner, C, chunks, P, vm);                                                             \
+    }
+        if (periodic) {
+            if (tab) VP_S(true, true) else VP_S(true, false)
+        } else {
+            if (tab) VP_S(false, true) else VP_S(false, false)
+        }
+#undef VP_S
     } else {
         int L = n >= 4096 ? 1 : static_cast<int>(4096 / n);
         if (L > outer) L = static_cast<int>(outer);
+        while (L > 1 && static_cast<int64_t>(L) * (n * sizeof(T) + tab_line) > kMaxLds) L /= 2;
+        while (L > 1 && (outer + L - 1) / L < 2 * cus) L /= 2;
+        VP_RETURN_IF(static_cast<int64_t>(L) * (n * sizeof(T) + tab_line) > kMaxLds, VP_ERR_UNSUPPORTED);
         const int64_t blocks = (outer + L - 1) / L;
         VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
-        const size_t lds = static_cast<size_t>(L) * n * sizeof(T);
-        if (periodic)
-            hipLaunchKernelGGL((advect_contig_kernel<T, true, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
-                               outer, n, L, P, vm);
-        else
-            hipLaunchKernelGGL((advect_contig_kernel<T, false, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout,
-                               outer, n, L, P, vm);
+        const size_t lds = static_cast<size_t>(L) * (n * sizeof(T) + tab_line);
+#define VP_C(PER, TB)                                                                                               \
+    {                                                                                                               \
+        static bool attr = false;                                                                                   \
+        if (!attr) {                                                                                                \
+            VP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&advect_contig_kernel<T, PER, FORM, TB>),  \
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kMaxLds))); \
+            attr = true;                                                                                            \
+        }                                                                                                           \
+        hipLaunchKernelGGL((advect_contig_kernel<T, PER, FORM, TB>), dim3(blocks), dim3(block), lds, s, fin, fout,  \
+                           outer, n, L, P, vm);                                                                     \
+    }
+        if (periodic) {
+            if (tab) VP_C(true, true) else VP_C(true, false)
+        } else {
+            if (tab) VP_C(false, true) else VP_C(false, false)
+        }
+#undef VP_C
     }
     VP_CHECK_LAUNCH();
     return VP_OK;

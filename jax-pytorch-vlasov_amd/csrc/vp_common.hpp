@@ -23,6 +23,12 @@
         if (cond) return (code); \
     } while (0)
 
+#define VP_TRY_HIP(expr)                                       \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return static_cast<int>(e_);     \
+    } while (0)
+
 #define VP_TRY(expr)                  \
     do {                              \
         int rc_ = (expr);             \
