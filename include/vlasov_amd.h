@@ -372,7 +372,15 @@ enum {
                                   workgroup (0 = 4)                            */
     VP_KNOB_ENS_XTHREADS = 8,  /* ensemble X drift: 256, 512 or 1024 threads
                                   per workgroup (0 = 512)                      */
-    VP_KNOB_COUNT = 9
+    VP_KNOB_SEG_ROWS = 9,      /* generic sweeps of strided lines longer than
+                                  256: 16..256 rows per workgroup segment
+                                  (0 = 128, fewer when that gives under
+                                  512 workgroups)                              */
+    VP_KNOB_SEG_CONTIG = 10,   /* generic sweeps of contiguous lines longer
+                                  than 256: 256..4096 indices per workgroup
+                                  segment (0 = 2048 for lines of 2048 or
+                                  more, else 1024)                             */
+    VP_KNOB_COUNT = 11
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

@@ -60,7 +60,7 @@ __device__ __forceinline__ void binade_weights(double disp, int b, T w[6]) {
     const double lo = ldexp(1.0, b);
     const double hi = 2.0 * lo;
     const double c0 = ceil(lo + disp);
-    for (int d = -1; d <= 1; ++d) {
+    for (int d = -1; d <= 2; ++d) {
         const double t = (c0 + d) - disp;  // the kernel's (double)r - disp for r = c0 + d
         if (t >= lo && t < hi) {
             weights_as<T>(t - floor(t), w);
@@ -71,17 +71,59 @@ __device__ __forceinline__ void binade_weights(double disp, int b, T w[6]) {
     for (int k = 0; k < 6; ++k) w[k] = T(0);  // no index of this line reaches binade b
 }
 
-// Fill tab[line][b][6] for `lines` lines (velocities vel_of(line)) with the
-// whole workgroup.
+// The same for negative departure points, t in (-2^(b+1), -2^b]: the
+// binade's grid is again uniform with r on it, so alpha depends on (b, disp)
+// only.
+template <typename T>
+__device__ __forceinline__ void binade_weights_neg(double disp, int b, T w[6]) {
+    const double lo = ldexp(1.0, b);
+    const double hi = 2.0 * lo;
+    const double c0 = floor(disp - lo);
+    for (int d = 1; d >= -2; --d) {
+        const double t = (c0 + d) - disp;
+        if (-t >= lo && -t < hi) {
+            weights_as<T>(t - floor(t), w);
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = T(0);
+}
+
+// Signed binade code of a departure point: e - 1 for t >= 1, -e for t <= -1
+// (|t| = m 2^e, m in [0.5, 1)); codes rise with t.  |t| < 1 or >= 2^kBinades
+// has no code (kNoCode).
+constexpr int kNoCode = 1 << 20;
+__device__ __forceinline__ int binade_code(double t) {
+    int e;
+    frexp(t, &e);
+    if (t >= 1.0 && e <= kBinades) return e - 1;
+    if (t <= -1.0 && e <= kBinades) return -e;
+    return kNoCode;
+}
+
+template <typename T>
+__device__ __forceinline__ void code_weights(double disp, int q, T w[6]) {
+    if (q >= 0)
+        binade_weights<T>(disp, q, w);
+    else
+        binade_weights_neg<T>(disp, -q - 1, w);
+}
+
+// Fill tab[line * ls + b * 6 + k] for `lines` lines (velocities
+// vel_of(line)) with the whole workgroup.  Lanes that read the tables of
+// different lines at once (strided sweeps) use an odd line stride ls, so
+// that they fall in different LDS banks.
+constexpr int kTabStride = kBinades * 6 + 1;
 template <typename T, typename VelOf>
-__device__ __forceinline__ void fill_binade_tables(T* tab, int lines, VelOf vel_of, double dt, double dx) {
+__device__ __forceinline__ void fill_binade_tables(T* tab, int lines, int ls, VelOf vel_of, double dt, double dx) {
     for (int e = threadIdx.x; e < lines * kBinades; e += blockDim.x) {
         const int line = e / kBinades, b = e % kBinades;
         const double disp = (vel_of(line) * dt) / dx;
         T w[6];
         binade_weights<T>(disp, b, w);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) tab[e * 6 + k] = w[k];
+        for (int k = 0; k < 6; ++k) tab[line * ls + b * 6 + k] = w[k];
     }
 }
 
@@ -219,7 +261,7 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
     T* tab = tile + static_cast<int64_t>(n) * C;
     if (TAB) {
         const int64_t lim = inner - i0;
-        fill_binade_tables<T>(tab, C, [&](int line) {
+        fill_binade_tables<T>(tab, C, kTabStride, [&](int line) {
             return line < lim ? P.vel[ooff + split_offset(i0 + line, vm.id, vm.is, vm.ni)] : 0.0;
         }, P.dt, P.dx);
     }
@@ -230,13 +272,324 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
     const int64_t voff = ooff + split_offset(i0 + c, vm.id, vm.is, vm.ni);
     if (TAB) {
         const double disp = (P.vel[voff] * P.dt) / P.dx;
-        const T* mytab = tab + c * (kBinades * 6);
+        const T* mytab = tab + c * kTabStride;
         for (int r = r0; r < n; r += rstep)
             fout[base + r * inner] = departure_value_tab<T, PERIODIC>(tile, r, disp, mytab, n, C, c);
         return;
     }
     for (int r = r0; r < n; r += rstep)
         fout[base + r * inner] = sweep_value<T, PERIODIC, FORM>(tile, r, voff + r * vm.s_line, P, n, C, c);
+}
+
+// ---------------------------------------------------------------------------
+// Long lines (n > 256) whose sweep parameters are constant along each line
+// (velocity stride 0 along the swept axis): the line is cut into segments,
+// and a workgroup stages only the elements its segment's taps reach.
+// ---------------------------------------------------------------------------
+
+// Per-line coefficients.  kDeparture: the displacement disp = (v dt)/dx
+// (weights per element, from binade tables).  Shift forms: the shift sh and
+// the six weights, fixed for the line, and the taps the static window keeps
+// (Dirichlet lines).  key(): the departure point of index r is r - key()
+// for the first tap's row (both forms), which bounds a segment's taps.
+template <typename T, bool PERIODIC, int FORM>
+struct LineSweep {
+    double disp;
+    int64_t sh;
+    T w[6];
+    T keep[6];
+    __device__ __forceinline__ void init(int64_t poff, const SweepParams& P) {
+        disp = 0.0;
+        sh = 0;
+        if (FORM == kDeparture) {
+            disp = (P.vel[poff] * P.dt) / P.dx;
+            return;
+        }
+        if (FORM == kReflected) {
+            const double d = (P.vel[poff] * P.dt) / P.dx;
+            const double fl = floor(d);
+            weights_as<T>(d - fl, w);
+            sh = static_cast<int64_t>(fl);
+        } else {
+            sh = P.shift[poff];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[k] = static_cast<T>(P.W[k * P.wk + poff]);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int64_t m = sh + (k - 2);
+            keep[k] = (P.window < 0 || (m >= -(P.window + 2) && m <= P.window + 2)) ? T(1) : T(0);
+        }
+    }
+    // taps of index r read rows floor(r - key()) - 2 .. floor(r - key()) + 3
+    __device__ __forceinline__ double key() const {
+        return FORM == kDeparture ? disp : static_cast<double>(sh) + 1.0;
+    }
+    // Value of index r with the taps read from memory at src[j * stride]
+    // (the arithmetic of sweep_value / interp_from_lds).
+    __device__ __forceinline__ T value_global(const T* src, int64_t stride, int r, int n) const {
+        T wl[6];
+        int64_t jt;  // row of tap 0
+        int64_t dir;
+        if (FORM == kDeparture) {
+            const double dep = static_cast<double>(r) - disp;
+            const double fl = floor(dep);
+            weights_as<T>(dep - fl, wl);
+            jt = static_cast<int64_t>(fl) - 2;
+            dir = 1;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wl[k] = w[k];
+            jt = r - sh + 2;
+            dir = -1;
+        }
+        T v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int64_t j = jt + dir * k;
+            if (PERIODIC) {
+                v[k] = src[static_cast<int64_t>(wrap_index(j, n)) * stride];
+            } else {
+                const int64_t jc = j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
+                const bool ok = j >= 0 && j < n && (FORM == kDeparture || keep[k] != T(0));
+                v[k] = src[jc * stride] * (ok ? T(1) : T(0));
+            }
+        }
+        return tap_sum<T>(wl, v);
+    }
+    // Shift forms: value of index r from staged rows, jb = staged index of
+    // row r - sh (taps at jb + 2 .. jb - 3).
+    __device__ __forceinline__ T value_staged_shift(const T* tile, int jb, int stride) const {
+        T v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const T x = tile[(jb + 2 - k) * stride];
+            v[k] = PERIODIC ? x : x * keep[k];
+        }
+        return tap_sum<T>(w, v);
+    }
+};
+
+// Departure form: weights of index r from a segment's table of binade codes
+// [q0, q0 + nq) (tab[(code - q0) * ts6 + k], ts6 = 6), else computed.
+template <typename T>
+__device__ __forceinline__ void departure_weights(double dep, double fl, const T* tab, int q0, int nq, T w[6]) {
+    const int slot = binade_code(dep) - q0;
+    if (slot >= 0 && slot < nq) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = tab[slot * 6 + k];
+    } else {
+        weights_as<T>(dep - fl, w);
+    }
+}
+
+// Binade codes a segment of departure points [dlo, dhi] reaches, clipped to
+// the coded range: [q0, q0 + nq).
+__device__ __forceinline__ void segment_codes(double dlo, double dhi, int& q0, int& nq) {
+    int a = binade_code(dlo), b = binade_code(dhi);
+    if (a == kNoCode) a = dlo < -1.0 ? -kBinades : 0;
+    if (b == kNoCode) b = dhi > 1.0 ? kBinades - 1 : -1;
+    q0 = a;
+    nq = b - a + 1;
+}
+
+// Strided lines: a workgroup takes KC = 256 B of contiguous columns and a
+// segment of R rows and stages the rows its taps reach (the segment widened
+// by the columns' spread of key() and the stencil; wrapped for periodic
+// lines, zero outside [0, n) otherwise), and for the departure form each
+// column's weights for up to kSegBinades binade codes from the segment's
+// lowest departure point up (an odd column stride puts the columns' tables
+// in different LDS banks).  A segment whose rows would not fit reads its
+// taps from global memory.
+constexpr int kSegBinades = 8;
+
+template <typename T, bool PERIODIC, int FORM>
+__global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ fin, T* __restrict__ fout, int n,
+                                                         int64_t inner, int64_t n_chunks, int n_segs, int R,
+                                                         int rows_cap, SweepParams P, VelMap vm) {
+    constexpr int KC = 256 / static_cast<int>(sizeof(T));
+    constexpr int NG = 256 / KC;
+    constexpr int TS = FORM == kDeparture ? kSegBinades * 6 + 1 : 0;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* tab = reinterpret_cast<T*>(smem_raw);  // [KC][TS]
+    T* tile = tab + KC * TS;                    // [rows_cap][KC]
+    __shared__ double keys[KC];
+    __shared__ int span[4];  // first staged row, staged rows (0: read global), first code, codes
+    const int64_t blk = blockIdx.x;
+    const int64_t chunk = blk % n_chunks;
+    const int64_t rest = blk / n_chunks;
+    const int seg = static_cast<int>(rest % n_segs);
+    const int64_t o = rest / n_segs;
+    const int c = threadIdx.x % KC;
+    const int g = threadIdx.x / KC;
+    const int64_t i0 = chunk * KC;
+    const int64_t lim = inner - i0;
+    const int nc = lim < KC ? static_cast<int>(lim) : KC;
+    const bool cval = c < nc;
+    const int r0 = seg * R;
+    const int r1 = (r0 + R < n) ? r0 + R : n;
+    const int64_t ooff = split_offset(o, vm.od, vm.os, vm.no);
+    const int64_t base = o * static_cast<int64_t>(n) * inner + i0 + c;
+    LineSweep<T, PERIODIC, FORM> ls;
+    ls.disp = 0.0;
+    ls.sh = 0;
+    if (cval) ls.init(ooff + split_offset(i0 + c, vm.id, vm.is, vm.ni), P);  // columns past the array read nothing
+    if (g == 0) keys[c] = cval ? ls.key() : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double kmin = keys[0], kmax = keys[0];
+        for (int k = 1; k < nc; ++k) {
+            kmin = fmin(kmin, keys[k]);
+            kmax = fmax(kmax, keys[k]);
+        }
+        // departure points of rows [r0, r1) lie in [RN(r0 - kmax), RN(r1 - 1 - kmin)]
+        // (rounding is monotone); one row of margin each side
+        const double dlo = static_cast<double>(r0) - kmax;
+        const double lo = floor(dlo) - 3.0;
+        const double hi = floor(static_cast<double>(r1 - 1) - kmin) + 4.0;
+        const bool fits = hi - lo + 1.0 <= static_cast<double>(rows_cap) && lo > -1073741824.0 && hi < 1073741824.0;
+        span[0] = fits ? static_cast<int>(lo) : 0;
+        span[1] = fits ? static_cast<int>(hi - lo) + 1 : 0;
+        int q0, nq;
+        segment_codes(dlo, static_cast<double>(r1 - 1) - kmin, q0, nq);
+        span[2] = q0;
+        span[3] = nq < kSegBinades ? nq : kSegBinades;
+    }
+    __syncthreads();
+    const int lo = span[0];
+    const int cnt = span[1];
+    if (cnt == 0) {
+        if (!cval) return;
+        const T* src = fin + base;
+        for (int r = r0 + g; r < r1; r += NG) fout[base + static_cast<int64_t>(r) * inner] = ls.value_global(src, inner, r, n);
+        return;
+    }
+    const int q0 = span[2];
+    const int nq = span[3];
+    if (FORM == kDeparture) {
+        for (int e = threadIdx.x; e < KC * kSegBinades; e += blockDim.x) {
+            const int line = e / kSegBinades, q = e % kSegBinades;
+            T w[6];
+            if (q < nq) {
+                code_weights<T>(keys[line], q0 + q, w);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) w[k] = T(0);
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) tab[line * TS + q * 6 + k] = w[k];
+        }
+    }
+    for (int rr = g; rr < cnt; rr += NG) {
+        const int row = lo + rr;
+        T v = T(0);
+        if (cval) {
+            if (PERIODIC)
+                v = fin[base + static_cast<int64_t>(wrap_index(row, n)) * inner];
+            else if (row >= 0 && row < n)
+                v = fin[base + static_cast<int64_t>(row) * inner];
+        }
+        tile[rr * KC + c] = v;
+    }
+    __syncthreads();
+    if (!cval) return;
+    if (FORM == kDeparture) {
+        const T* mytab = tab + c * TS;
+        for (int r = r0 + g; r < r1; r += NG) {
+            const double dep = static_cast<double>(r) - ls.disp;
+            const double fl = floor(dep);
+            T w[6];
+            departure_weights<T>(dep, fl, mytab, q0, nq, w);
+            int j0 = static_cast<int>(fl) - 2 - lo;  // staged row of tap 0, in [1, cnt - 7] for finite disp
+            j0 = j0 < 0 ? 0 : (j0 > cnt - 6 ? cnt - 6 : j0);
+            T v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = tile[(j0 + k) * KC + c];
+            fout[base + static_cast<int64_t>(r) * inner] = tap_sum<T>(w, v);
+        }
+    } else {
+        const int sh = static_cast<int>(ls.sh);  // |sh| < 2^30 when the segment fits
+        for (int r = r0 + g; r < r1; r += NG) {
+            int jb = r - sh - lo;  // staged row of r - sh, in [4, cnt - 3]
+            jb = jb < 3 ? 3 : (jb > cnt - 3 ? cnt - 3 : jb);
+            fout[base + static_cast<int64_t>(r) * inner] = ls.value_staged_shift(tile + c, jb, KC);
+        }
+    }
+}
+
+// Contiguous lines: a workgroup takes a segment of S indices of one line.
+// The whole line shares one key(), so the segment's taps span at most S + 9
+// consecutive elements (wrapped, or zero outside [0, n)), staged in LDS with,
+// for the departure form, the line's weights for every binade code the
+// segment reaches (at most 2 kBinades).  An out-of-range key() reads the
+// taps from global memory.
+template <typename T, bool PERIODIC, int FORM>
+__global__ __launch_bounds__(256) void advect_cseg_kernel(const T* __restrict__ fin, T* __restrict__ fout, int n,
+                                                          int S, int n_segs, SweepParams P, VelMap vm) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* tile = reinterpret_cast<T*>(smem_raw);  // [S + 16]
+    __shared__ T tab[2 * kBinades * 6];
+    const int64_t line = blockIdx.x / n_segs;
+    const int seg = static_cast<int>(blockIdx.x - line * n_segs);
+    const int r0 = seg * S;
+    const int r1 = (r0 + S < n) ? r0 + S : n;
+    const T* src = fin + line * n;
+    T* dst = fout + line * n;
+    LineSweep<T, PERIODIC, FORM> ls;
+    ls.init(split_offset(line, vm.od, vm.os, vm.no), P);
+    const double key = ls.key();
+    const double dlo = static_cast<double>(r0) - key;
+    const double dhi = static_cast<double>(r1 - 1) - key;
+    const double lo_d = floor(dlo) - 3.0;
+    const double hi_d = floor(dhi) + 4.0;
+    const bool fits = hi_d - lo_d + 1.0 <= static_cast<double>(S + 16) && lo_d > -1073741824.0 && hi_d < 1073741824.0;
+    if (!fits) {
+        for (int r = r0 + static_cast<int>(threadIdx.x); r < r1; r += blockDim.x) dst[r] = ls.value_global(src, 1, r, n);
+        return;
+    }
+    const int lo = static_cast<int>(lo_d);
+    const int cnt = static_cast<int>(hi_d - lo_d) + 1;
+    int q0 = 0, nq = 0;
+    if (FORM == kDeparture) {
+        segment_codes(dlo, dhi, q0, nq);  // nq <= 2 kBinades
+        if (static_cast<int>(threadIdx.x) < nq) {
+            T w[6];
+            code_weights<T>(ls.disp, q0 + static_cast<int>(threadIdx.x), w);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) tab[threadIdx.x * 6 + k] = w[k];
+        }
+    }
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+        const int row = lo + idx;
+        T v = T(0);
+        if (PERIODIC)
+            v = src[wrap_index(row, n)];
+        else if (row >= 0 && row < n)
+            v = src[row];
+        tile[idx] = v;
+    }
+    __syncthreads();
+    if (FORM == kDeparture) {
+        for (int r = r0 + static_cast<int>(threadIdx.x); r < r1; r += blockDim.x) {
+            const double dep = static_cast<double>(r) - ls.disp;
+            const double fl = floor(dep);
+            T w[6];
+            departure_weights<T>(dep, fl, tab, q0, nq, w);
+            int j0 = static_cast<int>(fl) - 2 - lo;  // staged index of tap 0, in [1, cnt - 7]
+            j0 = j0 < 0 ? 0 : (j0 > cnt - 6 ? cnt - 6 : j0);
+            T v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = tile[j0 + k];
+            dst[r] = tap_sum<T>(w, v);
+        }
+    } else {
+        const int sh = static_cast<int>(ls.sh);
+        for (int r = r0 + static_cast<int>(threadIdx.x); r < r1; r += blockDim.x) {
+            int jb = r - sh - lo;
+            jb = jb < 3 ? 3 : (jb > cnt - 3 ? cnt - 3 : jb);
+            dst[r] = ls.value_staged_shift(tile, jb, 1);
+        }
+    }
 }
 
 // Tile = L whole contiguous lines of length n; wave w handles lines w, w+4, ...
@@ -252,7 +605,7 @@ __global__ __launch_bounds__(256) void advect_contig_kernel(
     const T* src = fin + l0 * n;
     T* tab = tile + static_cast<int64_t>(L) * n;
     if (TAB)
-        fill_binade_tables<T>(tab, static_cast<int>(nl), [&](int line) {
+        fill_binade_tables<T>(tab, static_cast<int>(nl), kBinades * 6, [&](int line) {
             return P.vel[split_offset(l0 + line, vm.od, vm.os, vm.no)];
         }, P.dt, P.dx);
     for (int64_t idx = threadIdx.x; idx < cnt; idx += blockDim.x) tile[idx] = src[idx];
@@ -302,10 +655,32 @@ static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis,
     const int block = 256;
     // binade tables: departure form with the velocity constant along each line
     const bool tab = FORM == kDeparture && vm.s_line == 0 && n < (1 << kBinades);
-    const size_t tab_line = tab ? static_cast<size_t>(kBinades) * 6 * sizeof(T) : 0;
+    const size_t tab_line = tab ? static_cast<size_t>(kTabStride) * sizeof(T) : 0;
     const int64_t kMaxLds = 96 * 1024;
     const int64_t cus = 256;  // spread small problems over at least ~2 workgroups per CU
-    if (inner > 1) {
+    // segmented kernels: long lines with the sweep parameters constant along each line
+    const bool seg_ok = vm.s_line == 0 && n > 256;
+    if (inner > 1 && seg_ok && inner >= 256 / static_cast<int64_t>(sizeof(T))) {
+        constexpr int KC = 256 / static_cast<int>(sizeof(T));
+        const int64_t chunks = (inner + KC - 1) / KC;
+        int R = knob(VP_KNOB_SEG_ROWS);  // rows per segment (0: 128, fewer for small problems, down to 32)
+        if (R < 16 || R > 256 || (R & (R - 1))) {
+            R = 128;
+            while (R > 32 && outer * chunks * ((n + R - 1) / R) < 2 * cus) R /= 2;
+        }
+        const int rows_cap = R + 16;
+        const int n_segs = (n + R - 1) / R;
+        const int64_t blocks = outer * chunks * n_segs;
+        VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
+        const size_t ts = FORM == kDeparture ? kSegBinades * 6 + 1 : 0;
+        const size_t lds = (static_cast<size_t>(KC) * ts + static_cast<size_t>(rows_cap) * KC) * sizeof(T);
+        if (periodic)
+            hipLaunchKernelGGL((advect_seg_kernel<T, true, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout, n,
+                               inner, chunks, n_segs, R, rows_cap, P, vm);
+        else
+            hipLaunchKernelGGL((advect_seg_kernel<T, false, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout, n,
+                               inner, chunks, n_segs, R, rows_cap, P, vm);
+    } else if (inner > 1) {
         int C = 64;
         while (C > 1 && C / 2 >= inner) C /= 2;
         while (C > 1 && static_cast<int64_t>(n) * C * sizeof(T) + C * tab_line > kMaxLds) C /= 2;
@@ -332,6 +707,20 @@ static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis,
             if (tab) VP_S(false, true) else VP_S(false, false)
         }
 #undef VP_S
+    } else if (inner == 1 && seg_ok) {
+        int S = knob(VP_KNOB_SEG_CONTIG);
+        if (S < 256 || S > 4096 || (S & (S - 1))) S = n >= 2048 ? 2048 : 1024;
+        while (S > 256 && S / 2 >= n) S /= 2;
+        const int n_segs = (n + S - 1) / S;
+        const int64_t blocks = outer * n_segs;
+        VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
+        const size_t lds = static_cast<size_t>(S + 16) * sizeof(T);
+        if (periodic)
+            hipLaunchKernelGGL((advect_cseg_kernel<T, true, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout, n,
+                               S, n_segs, P, vm);
+        else
+            hipLaunchKernelGGL((advect_cseg_kernel<T, false, FORM>), dim3(blocks), dim3(block), lds, s, fin, fout, n,
+                               S, n_segs, P, vm);
     } else {
         int L = n >= 4096 ? 1 : static_cast<int>(4096 / n);
         if (L > outer) L = static_cast<int>(outer);

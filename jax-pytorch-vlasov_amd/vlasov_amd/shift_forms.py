@@ -70,8 +70,22 @@ def precompute_x_drift(v, dt, dx, dtype):
     return blocks, W
 
 
+_SHIFT_CACHE: dict = {}
+
+
 def block_shifts(blocks, device) -> torch.Tensor:
-    return torch.tensor([s for a, b, s in blocks for _ in range(b - a)], dtype=torch.int64, device=device)
+    """Per-line shifts of `blocks` as a device tensor.  The drivers pass the
+    same blocks every step, so the uploaded tensor is kept (a few recent
+    block lists per device): no host-to-device copy, which would stall the
+    stream, on the step path."""
+    key = (tuple(tuple(b) for b in blocks), str(torch.device(device)))
+    t = _SHIFT_CACHE.get(key)
+    if t is None:
+        if len(_SHIFT_CACHE) >= 16:
+            _SHIFT_CACHE.pop(next(iter(_SHIFT_CACHE)))
+        t = torch.tensor([s for a, b, s in blocks for _ in range(b - a)], dtype=torch.int64, device=device)
+        _SHIFT_CACHE[key] = t
+    return t
 
 
 def static_sweep(f4: torch.Tensor, axis: int, shifts: torch.Tensor, W: torch.Tensor, param_strides) -> torch.Tensor:

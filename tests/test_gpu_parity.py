@@ -73,6 +73,28 @@ def test_advect_broadcast_velocities_bit_identical(device, axis):
     np.testing.assert_array_equal(out, ref)
 
 
+@pytest.mark.parametrize("shape,axis", [((700, 3, 37, 2), 0), ((2, 3, 600, 40), 2), ((1, 1, 4096, 33), 2),
+                                        ((1, 1000, 64, 1), 1), ((3, 2, 5, 300), 3), ((1, 1, 7, 4096), 3),
+                                        ((1, 9, 2, 2500), 3)])
+@pytest.mark.parametrize("spread", [0.3, 4.0, 60.0, 3000.0])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_advect_long_lines(device, shape, axis, spread, periodic):
+    """Lines longer than 256, strided (row segments staged with their taps'
+    halo; a displacement spread wider than the halo reads taps from global
+    memory) and contiguous (segments of one line)."""
+    v2 = _v2()
+    rng = np.random.default_rng(int(shape[axis]) + int(spread * 10))
+    f = rng.random(shape)
+    dt, dx = 0.1, 0.25
+    batch = [1 if i == axis else s for i, s in enumerate(shape)]
+    vel = rng.uniform(-1, 1, size=batch) * spread * dx / dt
+    vel = vel.reshape([s for i, s in enumerate(batch) if i != axis])
+    ref = vo.advect_axis(f, vel, axis, dt, dx, periodic=periodic)
+    out = v2.advect_1d(torch.from_numpy(f).to(device), torch.from_numpy(vel).to(device), axis, dt, dx,
+                       periodic=periodic).cpu().numpy()
+    np.testing.assert_array_equal(out, ref)
+
+
 @pytest.mark.parametrize("periodic", [True, False])
 def test_advect_rows_per_point_velocity(device, periodic):
     v1 = _v1()

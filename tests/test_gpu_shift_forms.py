@@ -68,6 +68,30 @@ def test_1d_static_forms(device, S):
     np.testing.assert_array_equal(r.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("nx,nv", [(600, 70), (4096, 40), (40, 700), (9, 4096)])
+@pytest.mark.parametrize("vmax", [0.5, 12.0, 900.0])
+def test_1d_shift_forms_long_lines(device, nx, nv, vmax):
+    """Lines longer than 256 (segmented kernels, and their global-memory
+    fallback when the shifts spread wider than a segment's halo)."""
+    from vlasov_amd import vlasov1D1V as v1
+    f = _f((nx, nv), nx + nv)
+    tf = torch.from_numpy(f).to(device)
+    rng = np.random.default_rng(nv)
+    dt, dx = 0.1, 0.2
+    v = rng.uniform(-1, 1, nv) * vmax * dx / dt
+    E = rng.uniform(-1, 1, nx) * vmax * dx / dt
+    so = torch.arange(-2, 4)
+    a = v1.advect_1d_matrix_diag(tf, torch.from_numpy(v).to(device), dt, dx, so, periodic=True)
+    np.testing.assert_array_equal(a.cpu().numpy(), vo.advect_rows_reflected(f.T, v, dt, dx, True).T)
+    b = v1.advect_1d_matrix_diag(tf, torch.from_numpy(E).to(device), dt, dx, so, periodic=False)
+    np.testing.assert_array_equal(b.cpu().numpy(), vo.advect_rows_reflected(f, E, dt, dx, False))
+    c = v1.advect_v_static(tf, torch.from_numpy(E).to(device), dt, dx, 2, so)
+    np.testing.assert_array_equal(c.cpu().numpy(), vo.advect_rows_reflected(f, E, dt, dx, False, window=2))
+    blocks, W = v1.precompute_x_drift(torch.from_numpy(np.sort(v)).to(device), dt, dx, torch.float64)
+    d = v1.advect_x_static(tf, W, blocks, so)
+    np.testing.assert_array_equal(d.cpu().numpy(), vo.advect_x_static(f, W.cpu().numpy(), blocks))
+
+
 def test_1d_steps(device):
     from vlasov_amd import vlasov1D1V as v1
     g = v1.Grid.create(64, 96, 4 * math.pi, 6.0, device=device)
