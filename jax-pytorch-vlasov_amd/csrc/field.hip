@@ -6,7 +6,8 @@
 //
 // Poisson: the spatial grids are small (<= 512 per axis, 128 in the
 // benchmark configs), so the transforms are exact-twiddle float64 DFTs,
-// one line per workgroup, twiddles from sincospi.  Conventions follow
+// one line per workgroup, twiddles from sincospi; the 1D solve of a
+// power-of-two grid uses radix-2 FFTs in one workgroup.  Conventions follow
 // torch.fft / numpy.fft: rfft, then c2c along x; the inverse is c2c along x
 // (scaled 1/nx), then c2r along y (scaled 1/ny) reading only the real parts
 // of the DC and Nyquist bins.
@@ -124,6 +125,85 @@ __global__ __launch_bounds__(256) void p1d_inv_kernel(const double2* __restrict_
     const double inv_n = 1.0 / static_cast<double>(nx);
     E[x] = se * inv_n;
     phi[x] = sp * inv_n;
+}
+
+// 1D, power-of-two nx (8 to 4096): one workgroup, radix-2 FFTs in LDS.
+// Forward: the complex FFT of rho (bit-reversed load, decimation in time)
+// gives X_k; E_k = -i X_k inv_kx[k], P_k = X_k inv_kx_sq[k] for
+// k <= nx/2.  Both inverses at once: E and phi are real, so their
+// Hermitian spectra combine into Z = E + i P (Z_k = E_k + i P_k,
+// Z_{nx-k} = conj(E_k) + i conj(P_k), and only the real parts of the DC and
+// Nyquist bins, as irfft reads them), whose inverse FFT is e + i p.  Same
+// transform as the DFT kernels, O(nx log nx) instead of O(nx^2); agrees to
+// rounding.  Twiddles exp(-2 pi i j / nx), j < nx/2, from sincospi.
+constexpr int kFftMaxN = 4096;
+
+__device__ __forceinline__ void fft_radix2(double2* a, const double2* tw, int nx, bool inverse) {
+    const int half_n = nx >> 1;
+    for (int half = 1; half < nx; half <<= 1) {
+        const int step = half_n / half;  // twiddle stride of this stage
+        for (int t = threadIdx.x; t < half_n; t += blockDim.x) {
+            const int pos = t & (half - 1);
+            const int i = ((t - pos) << 1) + pos;  // group * 2 half + pos
+            const int j = i + half;
+            double2 w = tw[pos * step];
+            if (inverse) w.y = -w.y;
+            const double2 u = a[i];
+            const double2 b = a[j];
+            const double2 v = make_double2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
+            a[i] = make_double2(u.x + v.x, u.y + v.y);
+            a[j] = make_double2(u.x - v.x, u.y - v.y);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void poisson1d_fft_kernel(const double* __restrict__ rho, int nx, int lg,
+                                                             const double* __restrict__ inv_kx,
+                                                             const double* __restrict__ inv_kx_sq,
+                                                             double* __restrict__ E, double* __restrict__ phi) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* a = reinterpret_cast<double2*>(smem_raw);  // [nx]
+    double2* tw = a + nx;                                 // [nx / 2]
+    const int half_n = nx >> 1;
+    for (int j = threadIdx.x; j < half_n; j += blockDim.x) {
+        double sn, cs;
+        sincospi(2.0 * static_cast<double>(j) / static_cast<double>(nx), &sn, &cs);
+        tw[j] = make_double2(cs, -sn);
+    }
+    for (int x = threadIdx.x; x < nx; x += blockDim.x)
+        a[__brev(static_cast<unsigned>(x)) >> (32 - lg)] = make_double2(rho[x], 0.0);
+    __syncthreads();
+    fft_radix2(a, tw, nx, false);
+    // each thread reads bins k <= nx/2 it alone writes; bins above nx/2 are not read
+    for (int k = threadIdx.x; k <= half_n; k += blockDim.x) {
+        const double2 X = a[k];
+        const double ik = inv_kx[k], ik2 = inv_kx_sq[k];
+        const double er = X.y * ik, ei = -X.x * ik;
+        const double pr = X.x * ik2, pi = X.y * ik2;
+        if (k == 0 || k == half_n) {
+            a[k] = make_double2(er, pr);
+        } else {
+            a[k] = make_double2(er - pi, ei + pr);
+            a[nx - k] = make_double2(er + pi, pr - ei);
+        }
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        const int y = static_cast<int>(__brev(static_cast<unsigned>(x)) >> (32 - lg));
+        if (x < y) {
+            const double2 t = a[x];
+            a[x] = a[y];
+            a[y] = t;
+        }
+    }
+    __syncthreads();
+    fft_radix2(a, tw, nx, true);
+    const double inv_n = 1.0 / static_cast<double>(nx);  // exact for a power of two
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+        E[x] = a[x].x * inv_n;
+        phi[x] = a[x].y * inv_n;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -289,6 +369,22 @@ int poisson1d(const double* rho, int64_t nx, const double* inv_kx, const double*
     VP_RETURN_IF(!rho || !inv_kx || !inv_kx_sq || !E || !phi || !ws, VP_ERR_ARG);
     VP_RETURN_IF(nx < 1 || nx > 8192, VP_ERR_SHAPE);
     VP_RETURN_IF(ws_bytes < poisson1d_ws(nx), VP_ERR_WORKSPACE);
+    if (nx >= 8 && nx <= kFftMaxN && (nx & (nx - 1)) == 0) {
+        const int n = static_cast<int>(nx);
+        int lg = 0;
+        while ((1 << lg) < n) ++lg;
+        static bool attr = false;
+        if (!attr) {
+            VP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&poisson1d_fft_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           static_cast<int>(kFftMaxN / 2 * 3 * sizeof(double2))));
+            attr = true;
+        }
+        const size_t lds = static_cast<size_t>(n / 2 * 3) * sizeof(double2);
+        hipLaunchKernelGGL(poisson1d_fft_kernel, dim3(1), dim3(1024), lds, s, rho, n, lg, inv_kx, inv_kx_sq, E, phi);
+        VP_CHECK_LAUNCH();
+        return VP_OK;
+    }
     if (nx >= 512 && nx <= 4096) {
         // many workgroups; LDS: fwd tw + rho (96 KB at 4096), inv tw + Ek + Pk (128 KB)
         const int n = static_cast<int>(nx), nk = n / 2 + 1;
