@@ -299,8 +299,10 @@ def reference_configs(ms_2d_fp64: float, ms_2d_fp32) -> dict:
     """This library's per-step time on the configurations the reference
     published, next to those published times (other hardware).  2D 128^4
     comes from the main and FP32 legs; 2D 64^4 runs the fused runner, 1D1V
-    the drop-in step (vlasov1D1V.step_with_field) from Python, one call per
-    step, as the reference's own loop does."""
+    the drop-in steps from Python, one call per step, as the reference's own
+    loop does: solver 0 (vlasov1D1V.step_with_field) as `ms_per_step`, and
+    solvers 1 and 2, the ones the reference's times were measured with, in
+    `ms_per_step_by_solver` / `speedup_same_solver`."""
     import torch
 
     from vlasov_amd import configs, runner
@@ -320,18 +322,41 @@ def reference_configs(ms_2d_fp64: float, ms_2d_fp32) -> dict:
     run.advance(5)
     ours["2d2v_64_fp64"] = timed(run.advance, 128)
     del run
+    by_solver = {}
     for n, steps in ((128, 1000), (4096, 50)):
         g1 = v1.Grid.create(n, n, 4 * math.pi, 5.0, device="cuda")
         state = {"f": configs.landau_1d(g1, eps=1e-3, k=0.5)}
+        args = (g1.v, g1.inv_kx, g1.inv_kx_sq, g1.x_indices, g1.v_indices, g1.x_batch_offsets,
+                g1.v_batch_offsets, g1.stencil_offsets, g1.dx, g1.dv, g1.V_max, True)
+        blocks, W_x = v1.precompute_x_drift(g1.v, 0.025, g1.dx, torch.float64)
+        so2 = v1.static_stencil_offsets(g1.stencil_offsets)
 
-        def steps_1d(k):
+        def steps_s0(k):
             f = state["f"]
             for _ in range(k):
                 f, _, _, _ = v1.step_with_field(f, 0.05, g1.v, g1.inv_kx, g1.inv_kx_sq, g1.dx, g1.dv, True)
             state["f"] = f
 
-        steps_1d(5)
-        ours[f"1d1v_{n}_fp64"] = timed(steps_1d, steps)
+        def steps_s1(k):
+            f = state["f"]
+            for _ in range(k):
+                f, _, _ = v1.vlasov_step_matrix_diag(f, 0.05, *args)
+            state["f"] = f
+
+        def steps_s2(k):
+            # the driver's solver-2 loop: tables precomputed once, the core step per iteration
+            f = state["f"]
+            for _ in range(k):
+                f, _, _ = v1.vlasov_step_gather_free_impl(f, 0.05, W_x, blocks, so2, g1.inv_kx, g1.inv_kx_sq, g1.dv,
+                                                          True, 1)
+            state["f"] = f
+
+        times = {}
+        for sid, fn in ((0, steps_s0), (1, steps_s1), (2, steps_s2)):
+            fn(5)
+            times[sid] = timed(fn, steps)
+        ours[f"1d1v_{n}_fp64"] = times[0]
+        by_solver[f"1d1v_{n}_fp64"] = {f"solver{k}": v for k, v in times.items()}
     out = {}
     for key, refs in REFERENCE_TIMES.items():
         if ours.get(key) is None:
@@ -340,6 +365,11 @@ def reference_configs(ms_2d_fp64: float, ms_2d_fp32) -> dict:
         out[key] = {"ms_per_step": ours[key],
                     "reference": [{"impl": a, "hardware": b, "ms_per_step": c, "source": d} for a, b, c, d in refs],
                     "speedup_vs_fastest_reference": best / ours[key]}
+        if key in by_solver:
+            # the reference's times are its solver 1 (Kokkos) and solver 2 (PyTorch) steps
+            out[key]["ms_per_step_by_solver"] = by_solver[key]
+            out[key]["speedup_same_solver"] = {
+                f"{a} {b}": c / by_solver[key][f"solver{1 if 's1' in a else 2}"] for a, b, c, _ in refs}
     return out
 
 

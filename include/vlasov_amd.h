@@ -380,7 +380,10 @@ enum {
                                   than 256: 256..4096 indices per workgroup
                                   segment (0 = 2048 for lines of 2048 or
                                   more, else 1024)                             */
-    VP_KNOB_COUNT = 11
+    VP_KNOB_F32_PAIRS = 11,    /* float32 X drift: pairs of vy columns moved
+                                  as 8-byte elements, 1 = never, 2 = always
+                                  (0 = for the two-pass drift only)            */
+    VP_KNOB_COUNT = 12
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

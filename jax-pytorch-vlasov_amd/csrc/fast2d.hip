@@ -211,18 +211,49 @@ struct RingGeom {
     static constexpr int groups = kDriftThreads / col;
 };
 
+// Element of the LDS ring: one value of the storage type S (VW = 1), or a
+// pair of neighbouring float32 columns (VW = 2) that share their weights and
+// tap rows, moved as 8-byte loads and stores.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <typename S, int VW>
+struct LaneElem {
+    using type = S;
+};
+template <>
+struct LaneElem<float, 2> {
+    using type = f32x2;
+};
+
+template <typename S>
+__device__ __forceinline__ S tap_sum_e(const S (&w)[6], const S (&v)[6]) {
+    return tap_sum<S>(w, v);
+}
+// Each column of the pair: the scalar float32 sum, term for term.
+__device__ __forceinline__ f32x2 tap_sum_e(const float (&w)[6], const f32x2 (&v)[6]) {
+    float a[6], b[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        a[k] = v[k].x;
+        b[k] = v[k].y;
+    }
+    f32x2 o;
+    o.x = tap_sum<float>(w, a);
+    o.y = tap_sum<float>(w, b);
+    return o;
+}
+
 // Six taps of one row from a tile of n (power of two) rows of 256 bytes.
 // A = byte offset of (row il - 2 mod n, this thread's column); tap k is at
 // (A + 256 k) mod 256 n: two integer operations per tap from one register per
 // row, instead of six precomputed offsets (which the compiler would otherwise
 // hoist out of the tile loop: 6 registers per row).
-template <typename S, int N>
-__device__ __forceinline__ S ring_tap_sum(const S* tile, uint32_t A, const S (&w)[6]) {
+template <typename E, typename S, int N>
+__device__ __forceinline__ E ring_tap_sum(const E* tile, uint32_t A, const S (&w)[6]) {
     const char* base = reinterpret_cast<const char*>(tile);
-    S v[6];
+    E v[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = *reinterpret_cast<const S*>(base + ((A + 256u * k) & (256u * N - 1)));
-    return tap_sum<S>(w, v);
+    for (int k = 0; k < 6; ++k) v[k] = *reinterpret_cast<const E*>(base + ((A + 256u * k) & (256u * N - 1)));
+    return tap_sum_e(w, v);
 }
 
 // Same sweep as drift_kernel, but the tiles stream into an LDS ring of T
@@ -244,20 +275,30 @@ __device__ __forceinline__ S ring_tap_sum(const S* tile, uint32_t A, const S (&w
 // so the count stays the same to the end; for T = 2 the only load a wait
 // depends on is the one issued in the iteration before, and the last
 // iteration issues none.
-template <typename S, int R, int PASSES, bool RHO, bool LANE_W, int T, bool NT>
+// VW = 2 (float32 storage, weights shared along the tile row: the X drift):
+// each thread handles a pair of neighbouring columns as one 8-byte element,
+// with the float64 drift's thread layout; results are those of VW = 1.
+template <typename S, int R, int PASSES, bool RHO, bool LANE_W, int T, bool NT, int VW = 1>
 __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
-    const S* fin, S* fout, int nq, int nsplit, int64_t a_stride, int64_t row_stride,
+    const S* fin_s, S* fout_s, int nq, int nsplit, int64_t a_stride, int64_t row_stride,
     int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
     double2* __restrict__ partial) {
-    constexpr int KC = RingGeom<S>::col;
-    constexpr int KG = RingGeom<S>::groups;
+    using E = typename LaneElem<S, VW>::type;
+    static_assert(VW == 1 || (!RHO && !LANE_W), "column pairs share one weight set");
+    const E* fin = reinterpret_cast<const E*>(fin_s);
+    E* fout = reinterpret_cast<E*>(fout_s);
+    a_stride /= VW;  // strides in elements of E
+    row_stride /= VW;
+    loop_stride /= VW;
+    constexpr int KC = RingGeom<E>::col;
+    constexpr int KG = RingGeom<E>::groups;
     constexpr int n = R * KG;
     constexpr int P = n / 64;
-    constexpr int PER_LANE = 16 / static_cast<int>(sizeof(S));
+    constexpr int PER_LANE = 16 / static_cast<int>(sizeof(E));
     static_assert(P * 64 == n && P >= 1, "rows must be a multiple of 64");
     static_assert(T >= 2 && T <= 4, "ring of 2..4 tiles");  // the waits below cover T <= 4
-    __shared__ __align__(16) S ring[T][n * KC];
-    __shared__ __align__(16) S mid[PASSES == 2 ? n * KC : 1];  // first-pass result of the X(dt/2) pair
+    __shared__ __align__(16) E ring[T][n * KC];
+    __shared__ __align__(16) E mid[PASSES == 2 ? n * KC : 1];  // first-pass result of the X(dt/2) pair
     const int t = threadIdx.x;
     const int c = t & (KC - 1);
     const int g = t / KC;
@@ -271,7 +312,7 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     const int64_t base = static_cast<int64_t>(a) * a_stride + q * KC + c;
     const int j = LANE_W ? q * KC + c : a;
 
-    S w[R][6];
+    S w[R][6];  // weights (shared by the pair when VW = 2)
     uint32_t A[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
@@ -281,7 +322,7 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
         for (int k = 0; k < 6; ++k) wd[k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
         round_weights<S>(wd, w[i]);
         const int il = IL[static_cast<int64_t>(r) * nvel + j];
-        A[i] = static_cast<uint32_t>(((il - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(S)));
+        A[i] = static_cast<uint32_t>(((il - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(E)));
     }
     // Consume the weight loads here: otherwise the wait-count analysis of the
     // loop body also waits for the in-flight stores before each FMA.
@@ -297,7 +338,7 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
 
     // Load instruction k of this wave covers tile rows 4m .. 4m+3, m = wave + 16k;
     // lane -> row 4m + lane/16, the 16 bytes at column PER_LANE*(lane%16).
-    const S* src_lane[P];
+    const E* src_lane[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const int row = 4 * (wave + 16 * k) + (lane >> 4);
@@ -337,21 +378,21 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
             const int ln = l + T - 1 < l1 ? l + T - 1 : l1 - 1;
             issue(ln, (i + T - 1) % T);
         }
-        const S* tile = ring[slot];
+        const E* tile = ring[slot];
 #pragma unroll
         for (int ii = 0; ii < R; ++ii) asm volatile("" : "+v"(A[ii]));  // keep the tap offsets in the loop
         if (PASSES == 2) {
             // first pass into `mid` (free since the barrier above), second from it
 #pragma unroll
-            for (int ii = 0; ii < R; ++ii) mid[(g + KG * ii) * KC + c] = ring_tap_sum<S, n>(tile, A[ii], w[ii]);
+            for (int ii = 0; ii < R; ++ii) mid[(g + KG * ii) * KC + c] = ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
             __syncthreads();
             tile = mid;
         }
 #pragma unroll
         for (int ii = 0; ii < R; ++ii) {
-            const S o = ring_tap_sum<S, n>(tile, A[ii], w[ii]);
+            const E o = ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
             store_f<NT>(&fout[base + (g + KG * ii) * row_stride + l * loop_stride], o);
-            if (RHO) {
+            if constexpr (RHO) {
                 if (COMP)
                     ksum_add(acc[ii], static_cast<double>(o));
                 else
@@ -360,7 +401,7 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
         }
     }
     wait_vmcnt<0>();  // no direct-to-LDS load may still be in flight when the workgroup ends
-    if (RHO) {
+    if constexpr (RHO) {
         // Row sums over the KC columns through LDS (the ring is free now):
         // every thread parks its R partial sums at [row][column], then
         // 1024 / n threads per row add KC * n / 1024 columns each and merge
@@ -893,7 +934,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
 // Host side
 // ---------------------------------------------------------------------------
 // Launch-shape knobs (vp_tune_set); 0 = automatic choice.
-static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0};
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
 
@@ -991,20 +1032,20 @@ int fast2d_tables(const int64_t shape[4], const Fast2DWs& L, const double* vx, c
 // covers the line length (64 or 128 rows), unless the DRIFT_DMA knob is 1 or
 // fin is not 16-byte aligned (float64 only); otherwise the register-prefetch
 // kernel (float64 only).  R = rows per thread.
-template <typename S, int R, int PASSES, bool RHO, bool LANE_W>
+template <typename S, int R, int PASSES, bool RHO, bool LANE_W, int VW = 1>
 static int launch_drift(const S* fin, S* fout, int64_t blocks, int nq, int nsplit, int64_t a_stride,
                         int64_t row_stride, int64_t loop_stride, int lt, const double* W, const int* IL, int nvel,
                         double2* partial, hipStream_t s) {
     constexpr bool f64 = sizeof(S) == 8;
-    constexpr int rows = R * RingGeom<S>::groups;
+    constexpr int rows = R * RingGeom<typename LaneElem<S, VW>::type>::groups;
     const int mode = knob(VP_KNOB_DRIFT_DMA);
     const bool nt = (nt_mask() & (LANE_W ? 2 : 1)) != 0;
     const bool aligned = (reinterpret_cast<uintptr_t>(fin) & 15) == 0;
     if constexpr (rows == 64 || rows == 128) {
         if ((mode != 1 || !f64) && aligned) {
-#define VP_D(T, NT)                                                                                               \
-    hipLaunchKernelGGL((drift_dma_kernel<S, R, PASSES, RHO, LANE_W, T, NT>), dim3(blocks), dim3(kDriftThreads), 0, \
-                       s, fin, fout, nq, nsplit, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial)
+#define VP_D(T, NT)                                                                                        \
+    hipLaunchKernelGGL((drift_dma_kernel<S, R, PASSES, RHO, LANE_W, T, NT, VW>), dim3(blocks), dim3(kDriftThreads), \
+                       0, s, fin, fout, nq, nsplit, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial)
             if (mode == 3) {
                 if (nt) VP_D(3, true); else VP_D(3, false);
             } else {
@@ -1039,11 +1080,22 @@ static int xdrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs&
     const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
     const int nvel = static_cast<int>(nvx), lt = static_cast<int>(ny);
     int rc = VP_ERR_UNSUPPORTED;
-#define VP_X(N, PV)                                                                                                \
-    if (nx == N && (N) % KG == 0 && (N) / KG >= 1) {                                                               \
-        constexpr int RV = (N) / KG > 0 ? (N) / KG : 1;                                                            \
-        rc = launch_drift<S, RV, PV, false, false>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride, lt, \
-                                                   L.Wx, L.ILx, nvel, nullptr, s);                                 \
+    // float32 storage: neighbouring vy columns share the X drift's weights, so a thread can take
+    // a pair of them (8-byte loads and stores, the float64 thread layout).  Measured at 128^4:
+    // 4 % faster for the two-pass drift, 6 % slower for one pass; the knob: 1 never, 2 always.
+    const int pk = knob(VP_KNOB_F32_PAIRS);
+    const bool pairs = sizeof(S) == 4 && pk != 1 && (pk == 2 || passes == 2) &&
+                       (reinterpret_cast<uintptr_t>(fout) & 7) == 0;
+#define VP_X(N, PV)                                                                                                  \
+    if (nx == N && (N) % KG == 0 && (N) / KG >= 1) {                                                                 \
+        constexpr int RV = (N) / KG > 0 ? (N) / KG : 1;                                                              \
+        constexpr int RV2 = (N) / (2 * KG) > 0 ? (N) / (2 * KG) : 1;                                                 \
+        if (pairs && (N) >= 64)                                                                                      \
+            rc = launch_drift<S, RV2, PV, false, false, 2>(fin, fout, blocks, nq, S_, a_stride, row_stride,          \
+                                                           loop_stride, lt, L.Wx, L.ILx, nvel, nullptr, s);          \
+        else                                                                                                         \
+            rc = launch_drift<S, RV, PV, false, false>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride, \
+                                                       lt, L.Wx, L.ILx, nvel, nullptr, s);                           \
     }
     if (passes == 2) {
         VP_X(128, 2) else VP_X(64, 2) else VP_X(32, 2)

@@ -121,6 +121,35 @@ def test_fp32_xdrift_two_passes_equal_two_calls(device):
     assert torch.equal(one, gen)
 
 
+@pytest.mark.parametrize("shape", [(128, 64, 32, 128), (64, 64, 32, 64)])
+@pytest.mark.parametrize("passes", [1, 2])
+def test_fp32_xdrift_column_pairs_equal_single_columns(device, shape, passes):
+    """The float32 X drift moving pairs of vy columns as 8-byte elements
+    gives the bits of the one-column-per-thread kernel."""
+    v2, runner, L = _mods()
+    G, f0 = _setup(shape, 6)
+    f = torch.from_numpy(f0).float().to(device)
+    sh = L.i64x4(shape)
+    lib = L.lib()
+    ws = torch.empty(lib.vp_fused_workspace(sh, 4), dtype=torch.uint8, device=device)
+    s = torch.cuda.current_stream().cuda_stream
+    L.check(lib.vp_fused_tables_f32(sh, G.vx.data_ptr(), G.vy.data_ptr(), 0.05, G.dx, G.dy, ws.data_ptr(),
+                                    ws.numel(), s), "tables")
+    outs = []
+    old = lib.vp_tune_get(L.KNOBS["f32_pairs"])
+    try:
+        for mode in (2, 1):
+            lib.vp_tune_set(L.KNOBS["f32_pairs"], mode)
+            o = torch.empty_like(f)
+            L.check(lib.vp_fused_xdrift_f32(f.data_ptr(), o.data_ptr(), sh, passes, ws.data_ptr(), ws.numel(), s),
+                    "xdrift")
+            outs.append(o)
+    finally:
+        lib.vp_tune_set(L.KNOBS["f32_pairs"], old)
+    assert torch.equal(outs[0], outs[1])
+    assert not torch.equal(outs[0], f)
+
+
 def test_fp32_rejects_in_place_kick_and_small_shapes(device):
     v2, runner, L = _mods()
     lib = L.lib()
