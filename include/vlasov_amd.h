@@ -132,6 +132,20 @@ int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, 
                      double dx, double dv, int physics_mode, double* E, double* rho,
                      double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
 
+/* vlasov_step_matrix_diag (P1:625-665, solver 1) and
+ * vlasov_step_gather_free_impl (P1:668-701, solver 2) in one call:
+ * X(dt/2), Poisson, V(dt, E), X(dt/2) with the shift forms.  x_shift and W_x
+ * both null: X sweeps as advect_1d_matrix_diag (P1:302-337) from v; both
+ * given: X sweeps as advect_x_static (P1:408-448) with shift x_shift[j] and
+ * weights W_x[k * w_stride + j] for v column j.  The V sweep is
+ * advect_1d_matrix_diag (window < 0) or advect_v_static (P1:479-501, window
+ * S >= 0).  Workspace: vp_step_1d1v_workspace. */
+int vp_step_1d1v_shift_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
+                           const double* v, const int64_t* x_shift, const double* W_x, int64_t w_stride,
+                           const double* inv_kx, const double* inv_kx_sq, double dx, double dv,
+                           int physics_mode, int window, double* E, double* rho, double* phi, void* ws,
+                           size_t ws_bytes, vp_stream_t stream);
+
 /* vlasov_step_impl (P2:943-998): X, Y (dt/2), Poisson, Vx, Vy (dt), Y, X (dt/2).
  * Ex, Ey, delta_rho, phi: [nx][ny] outputs; zero when physics_mode == 0.
  * For nx, ny, nvx, nvy in {32, 64, 128} the step runs the fused in-place kernels

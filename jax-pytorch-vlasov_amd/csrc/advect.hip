@@ -240,53 +240,6 @@ __device__ __forceinline__ T sweep_value(const T* tile, int r, int64_t poff, con
     return tap_sum<T>(w, v);
 }
 
-// Tile = n rows (the swept axis) x C columns (contiguous inner elements).
-// TAB (departure form, velocity constant along each line): binade weight
-// tables of the C lines after the tile in LDS.
-template <typename T, bool PERIODIC, int FORM, bool TAB>
-__global__ __launch_bounds__(256) void advect_strided_kernel(
-    const T* __restrict__ fin, T* __restrict__ fout, int n, int64_t inner, int C,
-    int64_t n_chunks, SweepParams P, VelMap vm) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* tile = reinterpret_cast<T*>(smem_raw);
-    const int64_t blk = blockIdx.x;
-    const int64_t o = blk / n_chunks;
-    const int64_t i0 = (blk - o * n_chunks) * C;
-    const int c = threadIdx.x % C;
-    const int r0 = threadIdx.x / C;
-    const int rstep = blockDim.x / C;
-    const bool cval = (i0 + c) < inner;
-    const int64_t base = o * static_cast<int64_t>(n) * inner + i0 + c;
-    const int64_t ooff = split_offset(o, vm.od, vm.os, vm.no);
-    T* tab = tile + static_cast<int64_t>(n) * C;
-    if (TAB) {
-        const int64_t lim = inner - i0;
-        fill_binade_tables<T>(tab, C, kTabStride, [&](int line) {
-            return line < lim ? P.vel[ooff + split_offset(i0 + line, vm.id, vm.is, vm.ni)] : 0.0;
-        }, P.dt, P.dx);
-    }
-    if (cval)
-        for (int r = r0; r < n; r += rstep) tile[r * C + c] = fin[base + r * inner];
-    __syncthreads();
-    if (!cval) return;
-    const int64_t voff = ooff + split_offset(i0 + c, vm.id, vm.is, vm.ni);
-    if (TAB) {
-        const double disp = (P.vel[voff] * P.dt) / P.dx;
-        const T* mytab = tab + c * kTabStride;
-        for (int r = r0; r < n; r += rstep)
-            fout[base + r * inner] = departure_value_tab<T, PERIODIC>(tile, r, disp, mytab, n, C, c);
-        return;
-    }
-    for (int r = r0; r < n; r += rstep)
-        fout[base + r * inner] = sweep_value<T, PERIODIC, FORM>(tile, r, voff + r * vm.s_line, P, n, C, c);
-}
-
-// ---------------------------------------------------------------------------
-// Long lines (n > 256) whose sweep parameters are constant along each line
-// (velocity stride 0 along the swept axis): the line is cut into segments,
-// and a workgroup stages only the elements its segment's taps reach.
-// ---------------------------------------------------------------------------
-
 // Per-line coefficients.  kDeparture: the displacement disp = (v dt)/dx
 // (weights per element, from binade tables).  Shift forms: the shift sh and
 // the six weights, fixed for the line, and the taps the static window keeps
@@ -369,6 +322,60 @@ struct LineSweep {
         return tap_sum<T>(w, v);
     }
 };
+
+// Tile = n rows (the swept axis) x C columns (contiguous inner elements).
+// TAB (sweep parameters constant along each line): for the departure form,
+// binade weight tables of the C lines after the tile in LDS; for the shift
+// forms, each thread's line coefficients (shift, weights) computed once.
+template <typename T, bool PERIODIC, int FORM, bool TAB>
+__global__ __launch_bounds__(256) void advect_strided_kernel(
+    const T* __restrict__ fin, T* __restrict__ fout, int n, int64_t inner, int C,
+    int64_t n_chunks, SweepParams P, VelMap vm) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* tile = reinterpret_cast<T*>(smem_raw);
+    const int64_t blk = blockIdx.x;
+    const int64_t o = blk / n_chunks;
+    const int64_t i0 = (blk - o * n_chunks) * C;
+    const int c = threadIdx.x % C;
+    const int r0 = threadIdx.x / C;
+    const int rstep = blockDim.x / C;
+    const bool cval = (i0 + c) < inner;
+    const int64_t base = o * static_cast<int64_t>(n) * inner + i0 + c;
+    const int64_t ooff = split_offset(o, vm.od, vm.os, vm.no);
+    T* tab = tile + static_cast<int64_t>(n) * C;
+    if (TAB && FORM == kDeparture) {
+        const int64_t lim = inner - i0;
+        fill_binade_tables<T>(tab, C, kTabStride, [&](int line) {
+            return line < lim ? P.vel[ooff + split_offset(i0 + line, vm.id, vm.is, vm.ni)] : 0.0;
+        }, P.dt, P.dx);
+    }
+    if (cval)
+        for (int r = r0; r < n; r += rstep) tile[r * C + c] = fin[base + r * inner];
+    __syncthreads();
+    if (!cval) return;
+    const int64_t voff = ooff + split_offset(i0 + c, vm.id, vm.is, vm.ni);
+    if (TAB && FORM == kDeparture) {
+        const double disp = (P.vel[voff] * P.dt) / P.dx;
+        const T* mytab = tab + c * kTabStride;
+        for (int r = r0; r < n; r += rstep)
+            fout[base + r * inner] = departure_value_tab<T, PERIODIC>(tile, r, disp, mytab, n, C, c);
+        return;
+    }
+    if (TAB) {
+        LineSweep<T, PERIODIC, FORM> ls;
+        ls.init(voff, P);
+        for (int r = r0; r < n; r += rstep) fout[base + r * inner] = ls.value_global(tile + c, C, r, n);
+        return;
+    }
+    for (int r = r0; r < n; r += rstep)
+        fout[base + r * inner] = sweep_value<T, PERIODIC, FORM>(tile, r, voff + r * vm.s_line, P, n, C, c);
+}
+
+// ---------------------------------------------------------------------------
+// Long lines (n > 256) whose sweep parameters are constant along each line
+// (velocity stride 0 along the swept axis): the line is cut into segments,
+// and a workgroup stages only the elements its segment's taps reach.
+// ---------------------------------------------------------------------------
 
 // Departure form: weights of index r from a segment's table of binade codes
 // [q0, q0 + nq) (tab[(code - q0) * ts6 + k], ts6 = 6), else computed.
@@ -604,7 +611,7 @@ __global__ __launch_bounds__(256) void advect_contig_kernel(
     const int64_t cnt = nl * n;
     const T* src = fin + l0 * n;
     T* tab = tile + static_cast<int64_t>(L) * n;
-    if (TAB)
+    if (TAB && FORM == kDeparture)
         fill_binade_tables<T>(tab, static_cast<int>(nl), kBinades * 6, [&](int line) {
             return P.vel[split_offset(l0 + line, vm.od, vm.os, vm.no)];
         }, P.dt, P.dx);
@@ -617,10 +624,16 @@ __global__ __launch_bounds__(256) void advect_contig_kernel(
         const int64_t voff = split_offset(l0 + l, vm.od, vm.os, vm.no);
         const T* line = tile + static_cast<int64_t>(l) * n;
         T* dst = fout + (l0 + l) * n;
-        if (TAB) {
+        if (TAB && FORM == kDeparture) {
             const double disp = (P.vel[voff] * P.dt) / P.dx;
             const T* ltab = tab + l * (kBinades * 6);
             for (int j = lane; j < n; j += kWave) dst[j] = departure_value_tab<T, PERIODIC>(line, j, disp, ltab, n, 1, 0);
+            continue;
+        }
+        if (TAB) {
+            LineSweep<T, PERIODIC, FORM> ls;
+            ls.init(voff, P);
+            for (int j = lane; j < n; j += kWave) dst[j] = ls.value_global(line, 1, j, n);
             continue;
         }
         for (int j = lane; j < n; j += kWave)
@@ -653,9 +666,10 @@ static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis,
     }
     vm.s_line = vs[axis];
     const int block = 256;
-    // binade tables: departure form with the velocity constant along each line
-    const bool tab = FORM == kDeparture && vm.s_line == 0 && n < (1 << kBinades);
-    const size_t tab_line = tab ? static_cast<size_t>(kTabStride) * sizeof(T) : 0;
+    // sweep parameters constant along each line: binade tables (departure form)
+    // or per-line shift and weights (shift forms)
+    const bool tab = vm.s_line == 0 && (FORM != kDeparture || n < (1 << kBinades));
+    const size_t tab_line = tab && FORM == kDeparture ? static_cast<size_t>(kTabStride) * sizeof(T) : 0;
     const int64_t kMaxLds = 96 * 1024;
     const int64_t cus = 256;  // spread small problems over at least ~2 workgroups per CU
     // segmented kernels: long lines with the sweep parameters constant along each line

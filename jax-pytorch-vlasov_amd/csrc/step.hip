@@ -81,6 +81,51 @@ extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, i
     return VP_OK;
 }
 
+// Solver-1 and solver-2 steps in one call: the X sweeps are the reflected
+// periodic form from v at dt/2 (x_shift == W_x == null, solver 1) or the
+// static form with the given per-column shifts and weights W_x[k * w_stride
+// + j] (solver 2); the V sweep is the reflected Dirichlet form from E at dt
+// with the static window `window` (< 0: every tap, solver 1).  Same
+// workspace as vp_step_1d1v_f64.
+extern "C" int vp_step_1d1v_shift_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
+                                      const double* v, const int64_t* x_shift, const double* W_x, int64_t w_stride,
+                                      const double* inv_kx, const double* inv_kx_sq, double dx, double dv,
+                                      int physics_mode, int window, double* E, double* rho, double* phi, void* ws,
+                                      size_t ws_bytes, vp_stream_t stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VP_RETURN_IF(!f_in || !f_out || !E || !rho || !phi || !ws, VP_ERR_ARG);
+    VP_RETURN_IF((x_shift == nullptr) != (W_x == nullptr), VP_ERR_ARG);
+    VP_RETURN_IF(!x_shift && !v, VP_ERR_ARG);
+    VP_RETURN_IF(x_shift && w_stride < nv, VP_ERR_ARG);
+    VP_RETURN_IF(physics_mode && (!inv_kx || !inv_kx_sq), VP_ERR_ARG);
+    VP_RETURN_IF(ws_bytes < vp_step_1d1v_workspace(nx, nv), VP_ERR_WORKSPACE);
+    unsigned char* wb = static_cast<unsigned char*>(ws);
+    double* tmp = reinterpret_cast<double*>(wb);
+    wb += vp::align256(static_cast<size_t>(nx) * nv * sizeof(double));
+    wb += vp::align256(static_cast<size_t>(nx) * sizeof(double));
+    void* pws = wb;
+    const int64_t shape[4] = {1, 1, nx, nv};
+    const int64_t vs_x[4] = {0, 0, 0, 1};  // X sweep: parameters per v column
+    const int64_t vs_v[4] = {0, 0, 1, 0};  // V sweep: field per x row
+    const double half = dt / 2.0;
+    auto xsweep = [&](const double* a, double* b) -> int {
+        if (x_shift) return vp_advect_static_f64(a, b, shape, 2, x_shift, W_x, w_stride, vs_x, stream);
+        return vp_advect_reflected_f64(a, b, shape, 2, v, vs_x, half, dx, 1, -1, stream);
+    };
+    VP_TRY(xsweep(f_in, f_out));
+    if (physics_mode) {
+        VP_TRY(vp::row_sum<double>(f_out, nx, nv, dv, 1.0, 1.0, rho, s));
+        VP_TRY(vp::poisson1d(rho, nx, inv_kx, inv_kx_sq, E, phi, pws, vp::poisson1d_ws(nx), s));
+    } else {
+        VP_TRY(vp::fill(E, nx, 0.0, s));
+        VP_TRY(vp::fill(rho, nx, 0.0, s));
+        VP_TRY(vp::fill(phi, nx, 0.0, s));
+    }
+    VP_TRY(vp_advect_reflected_f64(f_out, tmp, shape, 3, E, vs_v, dt, dv, 0, window, stream));
+    VP_TRY(xsweep(tmp, f_out));
+    return VP_OK;
+}
+
 // ---------------------------------------------------------------------------
 // 2D2V: vlasov_step_impl (vlasov2D2V.py:943-998)
 // generic ws: [N^4] f scratch | [nx*ny] rho | poisson ws

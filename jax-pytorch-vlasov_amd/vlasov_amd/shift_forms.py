@@ -21,9 +21,20 @@ OFFSETS = (-2, -1, 0, 1, 2, 3)
 
 
 def static_stencil_offsets(stencil_offsets):
-    """Stencil offsets as a tuple of Python ints (vlasov1D1V.py:339-355)."""
+    """Stencil offsets as a tuple of Python ints (vlasov1D1V.py:339-355).
+    A device tensor is read once (reading it waits for the device): the
+    tuple is kept on the tensor object with its version counter, so the
+    per-step checks of the drivers do not synchronise."""
     if isinstance(stencil_offsets, torch.Tensor):
-        return tuple(int(o) for o in stencil_offsets.tolist())
+        cached = getattr(stencil_offsets, "_vp_static_offsets", None)
+        if cached is not None and cached[0] == stencil_offsets._version:
+            return cached[1]
+        t = tuple(int(o) for o in stencil_offsets.tolist())
+        try:
+            stencil_offsets._vp_static_offsets = (stencil_offsets._version, t)
+        except (AttributeError, RuntimeError):
+            pass
+        return t
     return tuple(int(o) for o in stencil_offsets)
 
 

@@ -352,23 +352,56 @@ def _field(f, physics_mode, inv_kx, inv_kx_sq, dv):
     return z, z.clone(), z.clone()
 
 
+def _shift_step(f, dt, v, x_shift, W_x, inv_kx, inv_kx_sq, dx, dv, physics_mode, window):
+    """One solver-1 (x_shift is None) or solver-2 step through
+    vp_step_1d1v_shift_f64: (f, rho, phi, E)."""
+    dev = require_gpu(f)
+    if f.dtype != torch.float64:
+        raise TypeError("the shift forms run float64")
+    f_c = f.contiguous()
+    Nx, Nv = f_c.shape
+    out = torch.empty_like(f_c)
+    E = torch.empty(Nx, dtype=torch.float64, device=dev)
+    rho = torch.empty(Nx, dtype=torch.float64, device=dev)
+    phi = torch.empty(Nx, dtype=torch.float64, device=dev)
+    ws = _lib.Workspace.get(dev, lib().vp_step_1d1v_workspace(Nx, Nv), "step1d")
+    pm = bool(physics_mode)
+    if x_shift is None:
+        v_c = _f64(v).contiguous()
+        sh_p, w_p, w_stride = None, None, 0
+    else:
+        v_c = None
+        W_c = W_x.to(torch.float64).contiguous()
+        sh_p, w_p, w_stride = x_shift.data_ptr(), W_c.data_ptr(), W_c.shape[1]
+    rc = lib().vp_step_1d1v_shift_f64(
+        f_c.data_ptr(), out.data_ptr(), Nx, Nv, float(dt), ptr(v_c) if v_c is not None else None, sh_p, w_p,
+        w_stride, ptr(inv_kx.contiguous()) if pm else None, ptr(inv_kx_sq.contiguous()) if pm else None,
+        float(dx), float(dv), int(pm), int(window), E.data_ptr(), rho.data_ptr(), phi.data_ptr(), ws.data_ptr(),
+        ws.numel(), stream_handle(dev))
+    check(rc, "vp_step_1d1v_shift_f64")
+    return out, rho, phi, E
+
+
 def vlasov_step_matrix_diag(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, x_indices, v_indices, x_batch_offsets,
                             v_batch_offsets, stencil_offsets, grid_dx, grid_dv, grid_V_max, physics_mode):
-    """Drop-in for vlasov_step_matrix_diag (P1:625-665), the solver-1 step."""
-    f = advect_1d_matrix_diag(f, grid_v, dt / 2.0, grid_dx, stencil_offsets, periodic=True)
-    E, rho, phi = _field(f, physics_mode, grid_inv_kx, grid_inv_kx_sq, grid_dv)
-    f = advect_1d_matrix_diag(f, E, dt, grid_dv, stencil_offsets, periodic=False)
-    f = advect_1d_matrix_diag(f, grid_v, dt / 2.0, grid_dx, stencil_offsets, periodic=True)
+    """Drop-in for vlasov_step_matrix_diag (P1:625-665), the solver-1 step:
+    advect_1d_matrix_diag X(dt/2), the field, V(dt, E), X(dt/2), in one
+    library call."""
+    _sf._check_offsets(stencil_offsets)
+    f, rho, phi, _ = _shift_step(f, dt, grid_v, None, None, grid_inv_kx, grid_inv_kx_sq, grid_dx, grid_dv,
+                                 physics_mode, -1)
     return f, rho, phi
 
 
 def vlasov_step_gather_free_impl(f, dt, W_x, blocks, stencil_offsets, grid_inv_kx, grid_inv_kx_sq, grid_dv,
                                  physics_mode, S):
-    """Drop-in for vlasov_step_gather_free_impl (P1:668-701), the solver-2 core."""
-    f = advect_x_static(f, W_x, blocks, stencil_offsets)
-    E, rho, phi = _field(f, physics_mode, grid_inv_kx, grid_inv_kx_sq, grid_dv)
-    f = advect_v_static(f, E, dt, grid_dv, S, stencil_offsets)
-    f = advect_x_static(f, W_x, blocks, stencil_offsets)
+    """Drop-in for vlasov_step_gather_free_impl (P1:668-701), the solver-2
+    core: advect_x_static, the field, advect_v_static, advect_x_static, in
+    one library call."""
+    _sf._check_offsets(stencil_offsets)
+    x_shift = _sf.block_shifts(blocks, f.device)
+    f, rho, phi, _ = _shift_step(f, dt, None, x_shift, W_x, grid_inv_kx, grid_inv_kx_sq, 1.0, grid_dv,
+                                 physics_mode, int(S))
     return f, rho, phi
 
 

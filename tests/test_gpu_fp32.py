@@ -7,8 +7,9 @@ float32; charge density, Poisson solve, fields and field energy in float64).
 * With the field on, fused and generic float32 runs differ only through the
   order of the float64 charge-density sums: <= 1e-6 relative.
 * Against the float64 oracle the float32 trajectory stays within 1e-5
-  relative in f and in the field-energy history (10 steps, 64x64x32x64).
+  relative in f and in the field-energy history (10 steps, 64x64x32x128).
 """
+import functools
 import math
 
 import numpy as np
@@ -72,24 +73,38 @@ def test_fp32_fused_matches_generic_with_field(device):
     assert np.max(np.abs(ea.cpu().numpy() - eb.cpu().numpy())) / np.max(eb.cpu().numpy()) <= 1e-6
 
 
-def test_fp32_trajectory_vs_fp64_oracle(device):
-    v2, runner, _ = _mods()
-    shape = (64, 64, 32, 64)
-    G, f0 = _setup(shape, 3)
-    g = vo.grid_2d(*shape, 4 * math.pi, 4 * math.pi, 6.0, 6.0)
+# Landau run at 64 x 64 x 32 x 128 (deviation storage needs 128 vy
+# columns), 10 steps of the float64 oracle, shared by the trajectory tests.
+_TRAJ_SHAPE = (64, 64, 32, 128)
+_TRAJ_STEPS = 10
+
+
+@functools.lru_cache(maxsize=1)
+def _oracle_trajectory():
+    """(f after the steps, field-energy history, last delta_rho) of the oracle."""
+    G, f0 = _setup(_TRAJ_SHAPE, 4)
+    g = vo.grid_2d(*_TRAJ_SHAPE, 4 * math.pi, 4 * math.pi, 6.0, 6.0)
     arrays = tuple(a.cpu().numpy() for a in (G.vx, G.vy, G.kx, G.ky, G.inv_ksq))
-    run = runner.Run2D(G, torch.from_numpy(f0).float().to(device), 0.1, physics_mode=True)
-    assert run.fused
-    energy = torch.zeros(10, dtype=torch.float64, device=device)
-    run.advance(10, energy=energy)
     f_ref = f0
     W = []
-    for _ in range(10):
-        f_ref, _, _, Ex, Ey = vo.vlasov_step_2d(f_ref, 0.1, g, True, grid_arrays=arrays)
+    drho = None
+    for _ in range(_TRAJ_STEPS):
+        f_ref, drho, _, Ex, Ey = vo.vlasov_step_2d(f_ref, 0.1, g, True, grid_arrays=arrays)
         W.append(vo.field_energy_2d(Ex, Ey, g.dx, g.dy))
+    return f_ref, np.array(W), drho
+
+
+def test_fp32_trajectory_vs_fp64_oracle(device):
+    v2, runner, _ = _mods()
+    G, f0 = _setup(_TRAJ_SHAPE, 4)
+    run = runner.Run2D(G, torch.from_numpy(f0).float().to(device), 0.1, physics_mode=True)
+    assert run.fused
+    energy = torch.zeros(_TRAJ_STEPS, dtype=torch.float64, device=device)
+    run.advance(_TRAJ_STEPS, energy=energy)
+    f_ref, W, _ = _oracle_trajectory()
     f32 = run.state.double().cpu().numpy()
     err_f = np.max(np.abs(f32 - f_ref)) / np.max(np.abs(f_ref))
-    err_w = np.max(np.abs(energy.cpu().numpy() - np.array(W))) / max(W)
+    err_w = np.max(np.abs(energy.cpu().numpy() - W)) / max(W)
     # the float32 input rounding alone is ~6e-8 relative
     assert err_f <= 1e-5, err_f
     assert err_w <= 1e-5, err_w
@@ -286,21 +301,14 @@ def test_deviation_trajectory_vs_fp64_oracle(device):
     the float64 oracle to 1e-6 in the field-energy history (the plain
     float32 form only to 1e-5 here), and mass to 1e-9."""
     v2, runner, _ = _mods()
-    shape = (64, 64, 32, 128)
-    G, f0 = _setup(shape, 4)
-    g = vo.grid_2d(*shape, 4 * math.pi, 4 * math.pi, 6.0, 6.0)
-    arrays = tuple(a.cpu().numpy() for a in (G.vx, G.vy, G.kx, G.ky, G.inv_ksq))
+    G, f0 = _setup(_TRAJ_SHAPE, 4)
     run = runner.Run2D(G, torch.from_numpy(f0).to(device), 0.1, physics_mode=True, storage="deviation")
-    energy = torch.zeros(10, dtype=torch.float64, device=device)
-    run.advance(10, energy=energy)
-    f_ref = f0
-    W = []
-    for _ in range(10):
-        f_ref, drho, _, Ex, Ey = vo.vlasov_step_2d(f_ref, 0.1, g, True, grid_arrays=arrays)
-        W.append(vo.field_energy_2d(Ex, Ey, g.dx, g.dy))
+    energy = torch.zeros(_TRAJ_STEPS, dtype=torch.float64, device=device)
+    run.advance(_TRAJ_STEPS, energy=energy)
+    f_ref, W, drho = _oracle_trajectory()
     f_dev = run.state_f64().cpu().numpy()
     err_f = np.max(np.abs(f_dev - f_ref)) / np.max(np.abs(f_ref))
-    err_w = np.max(np.abs(energy.cpu().numpy() - np.array(W))) / max(W)
+    err_w = np.max(np.abs(energy.cpu().numpy() - W)) / max(W)
     assert err_f <= 2e-6, err_f
     assert err_w <= 1e-6, err_w
     assert np.max(np.abs(run.delta_rho.cpu().numpy() - drho)) <= 1e-6

@@ -92,6 +92,46 @@ def test_1d_shift_forms_long_lines(device, nx, nv, vmax):
     np.testing.assert_array_equal(d.cpu().numpy(), vo.advect_x_static(f, W.cpu().numpy(), blocks))
 
 
+@pytest.mark.parametrize("n", [64, 600])
+@pytest.mark.parametrize("physics_mode", [True, False])
+def test_1d_one_call_steps_equal_composed_sweeps(device, n, physics_mode):
+    """The solver-1 and solver-2 steps (one library call each) equal the
+    reference's composition of the sweep and field functions, bit for bit."""
+    from vlasov_amd import vlasov1D1V as v1
+    g = v1.Grid.create(n, 96, 4 * math.pi, 6.0, device=device)
+    x = g.x.cpu().numpy()[:, None]
+    vv = g.v.cpu().numpy()[None, :]
+    f0 = torch.from_numpy((1 + 0.3 * np.cos(0.5 * x)) * np.exp(-0.5 * vv ** 2) / math.sqrt(2 * math.pi)).to(device)
+    so = g.stencil_offsets
+    dt = 0.3
+    args = (g.v, g.inv_kx, g.inv_kx_sq, g.x_indices, g.v_indices, g.x_batch_offsets, g.v_batch_offsets, so, g.dx,
+            g.dv, g.V_max, physics_mode)
+
+    def field(f):
+        if physics_mode:
+            return v1.solve_poisson_real(f, g.inv_kx, g.inv_kx_sq, g.dv)
+        z = torch.zeros(n, dtype=torch.float64, device=device)
+        return z, z, z
+
+    a, rho_a, phi_a = v1.vlasov_step_matrix_diag(f0, dt, *args)
+    f1 = v1.advect_1d_matrix_diag(f0, g.v, dt / 2, g.dx, so, periodic=True)
+    E, rho, phi = field(f1)
+    f2 = v1.advect_1d_matrix_diag(f1, E, dt, g.dv, so, periodic=False)
+    f3 = v1.advect_1d_matrix_diag(f2, g.v, dt / 2, g.dx, so, periodic=True)
+    assert torch.equal(a, f3) and torch.equal(rho_a, rho) and torch.equal(phi_a, phi)
+    blocks, W = v1.precompute_x_drift(g.v, dt / 2, g.dx, torch.float64)
+    sso = v1.static_stencil_offsets(so)
+    b, rho_b, phi_b = v1.vlasov_step_gather_free_impl(f0, dt, W, blocks, sso, g.inv_kx, g.inv_kx_sq, g.dv,
+                                                      physics_mode, 2)
+    h1 = v1.advect_x_static(f0, W, blocks, sso)
+    E, rho, phi = field(h1)
+    h2 = v1.advect_v_static(h1, E, dt, g.dv, 2, sso)
+    h3 = v1.advect_x_static(h2, W, blocks, sso)
+    assert torch.equal(b, h3) and torch.equal(rho_b, rho) and torch.equal(phi_b, phi)
+    if physics_mode:
+        assert float(E.abs().max()) > 0
+
+
 def test_1d_steps(device):
     from vlasov_amd import vlasov1D1V as v1
     g = v1.Grid.create(64, 96, 4 * math.pi, 6.0, device=device)
