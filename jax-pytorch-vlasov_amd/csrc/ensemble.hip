@@ -14,6 +14,7 @@
 //
 // Launch shapes: grid.y = instance; grid.x = column chunk (X drift) or row
 // block (kick); workgroups past an instance's extent exit at once.
+#include "binade.hpp"
 #include "field_dev.hpp"
 
 namespace vp {
@@ -21,16 +22,48 @@ namespace vp {
 constexpr int kEnsThreads = 256;
 constexpr int kEnsMaxNx = 256;
 constexpr int kEnsMaxNv = 512;
+// Binade codes tabulated per line (a line of at most 512 points reaches
+// about 10; departure points with other codes compute their weights).
+constexpr int kEnsCodes = 16;
+constexpr int kEnsTabStride = kEnsCodes * 6 + 1;  // odd: lines' tables in different LDS banks
+
+// Weight table of one line of n points with displacement disp: codes
+// [q[0], q[0] + q[1]) of tab[code slot * 6 + k], filled by fill_line_tab.
+__device__ __forceinline__ void line_codes(double disp, int n, int* q) {
+    int q0, nq;
+    segment_codes(-disp, static_cast<double>(n - 1) - disp, q0, nq);  // departure points of 0 and n - 1
+    q[0] = q0;
+    q[1] = nq < kEnsCodes ? nq : kEnsCodes;
+}
+
+// Every workgroup thread fills entries of `lines` tables (line l at
+// tab + l * kEnsTabStride, displacement disp[l], codes q[2 l], q[2 l + 1]).
+__device__ __forceinline__ void fill_line_tabs(double* tab, int lines, const double* disp, const int* q, int nt) {
+    for (int e = threadIdx.x; e < lines * kEnsCodes; e += nt) {
+        const int l = e / kEnsCodes, slot = e % kEnsCodes;
+        double w[6];
+        if (slot < q[2 * l + 1]) {
+            code_weights<double>(disp[l], q[2 * l] + slot, w);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) tab[l * kEnsTabStride + slot * 6 + k] = w[k];
+    }
+}
 
 // One periodic X-drift value: row r of column c of a tile of n (>= 6) rows x
-// C columns.  One int64 remainder per element, the six taps by stepping.
+// C columns, the departure form with the column's displacement and weight
+// table.  One int64 remainder per element, the six taps by stepping.
 template <int C>
-__device__ __forceinline__ double ens_periodic_sum(const double* tile, int r, double vel, double dt, double dx, int n,
-                                                   int c) {
-    const Departure d = departure(r, vel, dt, dx);
+__device__ __forceinline__ double ens_periodic_sum(const double* tile, int r, double disp, const double* tab,
+                                                   const int* q, int n, int c) {
+    const double dep = static_cast<double>(r) - disp;
+    const double fl = floor(dep);
     double w[6];
-    lagrange5_weights<double>(d.alpha, w);
-    int jr = wrap_index(d.il - 2, n);
+    departure_weights<double>(dep, fl, tab, q[0], q[1], w);
+    int jr = wrap_index(static_cast<int64_t>(fl) - 2, n);
     double v[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -51,6 +84,9 @@ __global__ __launch_bounds__(NT) void ens_xdrift_kernel(double* __restrict__ poo
                                                                  int maxch, double2* __restrict__ partial) {
     __shared__ double bufA[kEnsMaxNx * C];
     __shared__ double bufB[kEnsMaxNx * C];
+    __shared__ double tab[C * kEnsTabStride];
+    __shared__ double disps[C];
+    __shared__ int codes[2 * C];
     const vp_ens_inst I = inst[blockIdx.y];
     const int nx = I.nx, nv = I.nv;
     const int c0 = blockIdx.x * C;
@@ -63,13 +99,22 @@ __global__ __launch_bounds__(NT) void ens_xdrift_kernel(double* __restrict__ poo
     const bool valid = j < nv;
     double* f = pool + I.f_off;
     const double vel = valid ? tables[I.v_off + j] : 0.0;
+    const double disp = (vel * dt) / I.dx;  // departure(), vlasov1D1V.py:229-238
+    if (g == 0) {
+        disps[c] = disp;
+        line_codes(disp, nx, codes + 2 * c);
+    }
     for (int r = g; r < nx; r += NG) bufA[r * C + c] = valid ? f[static_cast<int64_t>(r) * nv + j] : 0.0;
     __syncthreads();
-    for (int r = g; r < nx; r += NG) bufB[r * C + c] = ens_periodic_sum<C>(bufA, r, vel, dt, I.dx, nx, c);
+    fill_line_tabs(tab, C, disps, codes, NT);
+    __syncthreads();
+    const double* mytab = tab + c * kEnsTabStride;
+    const int* myq = codes + 2 * c;
+    for (int r = g; r < nx; r += NG) bufB[r * C + c] = ens_periodic_sum<C>(bufA, r, disp, mytab, myq, nx, c);
     const double* fin = bufB;  // result of the last pass
     if (PASSES == 2) {
         __syncthreads();
-        for (int r = g; r < nx; r += NG) bufA[r * C + c] = ens_periodic_sum<C>(bufB, r, vel, dt, I.dx, nx, c);
+        for (int r = g; r < nx; r += NG) bufA[r * C + c] = ens_periodic_sum<C>(bufB, r, disp, mytab, myq, nx, c);
         fin = bufA;
     }
     __syncthreads();
@@ -138,6 +183,9 @@ __global__ __launch_bounds__(kEnsThreads) void ens_kick_kernel(double* __restric
                                                                const vp_ens_inst* __restrict__ inst,
                                                                const double* __restrict__ Ein, double dt) {
     __shared__ double tile[RK * kEnsMaxNv];
+    __shared__ double tab[RK * kEnsTabStride];
+    __shared__ double disps[RK];
+    __shared__ int codes[2 * RK];
     const vp_ens_inst I = inst[blockIdx.y];
     const int nx = I.nx, nv = I.nv;
     const int x0 = blockIdx.x * RK;
@@ -145,20 +193,29 @@ __global__ __launch_bounds__(kEnsThreads) void ens_kick_kernel(double* __restric
     const int rows = nx - x0 < RK ? nx - x0 : RK;
     double* f = pool + I.f_off + static_cast<int64_t>(x0) * nv;
     const int cnt = rows * nv;
+    const double* E = Ein + static_cast<int64_t>(blockIdx.y) * kEnsMaxNx + x0;
+    if (static_cast<int>(threadIdx.x) < rows) {
+        const double disp = (E[threadIdx.x] * dt) / I.dv;  // departure(), vlasov1D1V.py:229-238
+        disps[threadIdx.x] = disp;
+        line_codes(disp, nv, codes + 2 * threadIdx.x);
+    }
     for (int idx = threadIdx.x; idx < cnt; idx += kEnsThreads) tile[idx] = f[idx];
     __syncthreads();
-    const double* E = Ein + static_cast<int64_t>(blockIdx.y) * kEnsMaxNx + x0;
+    fill_line_tabs(tab, rows, disps, codes, kEnsThreads);
+    __syncthreads();
     for (int idx = threadIdx.x; idx < cnt; idx += kEnsThreads) {
         const int rr = idx / nv;
         const int j = idx - rr * nv;
-        const Departure d = departure(j, E[rr], dt, I.dv);
+        const double dep = static_cast<double>(j) - disps[rr];
+        const double fl = floor(dep);
         double w[6];
-        lagrange5_weights<double>(d.alpha, w);
+        departure_weights<double>(dep, fl, tab + rr * kEnsTabStride, codes[2 * rr], codes[2 * rr + 1], w);
+        const int64_t il = static_cast<int64_t>(fl);
         const double* row = tile + rr * nv;
         double v[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            const int64_t jj = d.il + (k - 2);
+            const int64_t jj = il + (k - 2);
             const int jc = static_cast<int>(jj < 0 ? 0 : (jj > nv - 1 ? nv - 1 : jj));
             const double m = (jj >= 0 && jj < nv) ? 1.0 : 0.0;
             v[k] = row[jc] * m;
