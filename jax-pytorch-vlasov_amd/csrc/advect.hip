@@ -313,6 +313,7 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
 // in different LDS banks).  A segment whose rows would not fit reads its
 // taps from global memory.
 constexpr int kSegBinades = 8;
+constexpr int kSegPrefetch = 8;  // staged elements per thread loaded before the tables are filled
 
 template <typename T, bool PERIODIC, int FORM>
 __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ fin, T* __restrict__ fout, int n,
@@ -377,6 +378,25 @@ __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ f
     }
     const int q0 = span[2];
     const int nq = span[3];
+    auto staged = [&](int rr) -> T {
+        const int row = lo + rr;
+        T v = T(0);
+        if (cval) {
+            if (PERIODIC)
+                v = fin[base + static_cast<int64_t>(wrap_index(row, n)) * inner];
+            else if (row >= 0 && row < n)
+                v = fin[base + static_cast<int64_t>(row) * inner];
+        }
+        return v;
+    };
+    // the first kSegPrefetch staged rows of this thread are loading while the
+    // weight tables are computed
+    T pre[kSegPrefetch];
+#pragma unroll
+    for (int k = 0; k < kSegPrefetch; ++k) {
+        const int rr = g + k * NG;
+        pre[k] = rr < cnt ? staged(rr) : T(0);
+    }
     if (FORM == kDeparture) {
         for (int e = threadIdx.x; e < KC * kSegBinades; e += blockDim.x) {
             const int line = e / kSegBinades, q = e % kSegBinades;
@@ -391,17 +411,12 @@ __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ f
             for (int k = 0; k < 6; ++k) tab[line * TS + q * 6 + k] = w[k];
         }
     }
-    for (int rr = g; rr < cnt; rr += NG) {
-        const int row = lo + rr;
-        T v = T(0);
-        if (cval) {
-            if (PERIODIC)
-                v = fin[base + static_cast<int64_t>(wrap_index(row, n)) * inner];
-            else if (row >= 0 && row < n)
-                v = fin[base + static_cast<int64_t>(row) * inner];
-        }
-        tile[rr * KC + c] = v;
+#pragma unroll
+    for (int k = 0; k < kSegPrefetch; ++k) {
+        const int rr = g + k * NG;
+        if (rr < cnt) tile[rr * KC + c] = pre[k];
     }
+    for (int rr = g + kSegPrefetch * NG; rr < cnt; rr += NG) tile[rr * KC + c] = staged(rr);
     __syncthreads();
     if (!cval) return;
     if (FORM == kDeparture) {
@@ -460,6 +475,21 @@ __global__ __launch_bounds__(256) void advect_cseg_kernel(const T* __restrict__ 
     }
     const int lo = static_cast<int>(lo_d);
     const int cnt = static_cast<int>(hi_d - lo_d) + 1;
+    auto staged = [&](int idx) -> T {
+        const int row = lo + idx;
+        T v = T(0);
+        if (PERIODIC)
+            v = src[wrap_index(row, n)];
+        else if (row >= 0 && row < n)
+            v = src[row];
+        return v;
+    };
+    T pre[kSegPrefetch];  // loading while the weight table is computed
+#pragma unroll
+    for (int k = 0; k < kSegPrefetch; ++k) {
+        const int idx = static_cast<int>(threadIdx.x) + k * static_cast<int>(blockDim.x);
+        pre[k] = idx < cnt ? staged(idx) : T(0);
+    }
     int q0 = 0, nq = 0;
     if (FORM == kDeparture) {
         segment_codes(dlo, dhi, q0, nq);  // nq <= 2 kBinades
@@ -470,15 +500,14 @@ __global__ __launch_bounds__(256) void advect_cseg_kernel(const T* __restrict__ 
             for (int k = 0; k < 6; ++k) tab[threadIdx.x * 6 + k] = w[k];
         }
     }
-    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-        const int row = lo + idx;
-        T v = T(0);
-        if (PERIODIC)
-            v = src[wrap_index(row, n)];
-        else if (row >= 0 && row < n)
-            v = src[row];
-        tile[idx] = v;
+#pragma unroll
+    for (int k = 0; k < kSegPrefetch; ++k) {
+        const int idx = static_cast<int>(threadIdx.x) + k * static_cast<int>(blockDim.x);
+        if (idx < cnt) tile[idx] = pre[k];
     }
+    for (int idx = static_cast<int>(threadIdx.x) + kSegPrefetch * static_cast<int>(blockDim.x); idx < cnt;
+         idx += blockDim.x)
+        tile[idx] = staged(idx);
     __syncthreads();
     if (FORM == kDeparture) {
         for (int r = r0 + static_cast<int>(threadIdx.x); r < r1; r += blockDim.x) {
