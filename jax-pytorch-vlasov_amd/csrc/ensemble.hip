@@ -36,14 +36,20 @@ __device__ __forceinline__ void line_codes(double disp, int n, int* q) {
     q[1] = nq < kEnsCodes ? nq : kEnsCodes;
 }
 
-// Every workgroup thread fills entries of `lines` tables (line l at
-// tab + l * kEnsTabStride, displacement disp[l], codes q[2 l], q[2 l + 1]).
-__device__ __forceinline__ void fill_line_tabs(double* tab, int lines, const double* disp, const int* q, int nt) {
+// Every workgroup thread fills entries of `lines` tables of lines of n
+// points (line l at tab + l * kEnsTabStride, displacement disp_of(l), which
+// each thread evaluates for the lines it fills: no barrier is needed before
+// the fill, so loads issued before it stay in flight).
+template <typename DispOf>
+__device__ __forceinline__ void fill_line_tabs(double* tab, int lines, int n, DispOf disp_of, int nt) {
     for (int e = threadIdx.x; e < lines * kEnsCodes; e += nt) {
         const int l = e / kEnsCodes, slot = e % kEnsCodes;
+        const double disp = disp_of(l);
+        int q[2];
+        line_codes(disp, n, q);
         double w[6];
-        if (slot < q[2 * l + 1]) {
-            code_weights<double>(disp[l], q[2 * l] + slot, w);
+        if (slot < q[1]) {
+            code_weights<double>(disp, q[0] + slot, w);
         } else {
 #pragma unroll
             for (int k = 0; k < 6; ++k) w[k] = 0.0;
@@ -85,8 +91,6 @@ __global__ __launch_bounds__(NT) void ens_xdrift_kernel(double* __restrict__ poo
     __shared__ double bufA[kEnsMaxNx * C];
     __shared__ double bufB[kEnsMaxNx * C];
     __shared__ double tab[C * kEnsTabStride];
-    __shared__ double disps[C];
-    __shared__ int codes[2 * C];
     const vp_ens_inst I = inst[blockIdx.y];
     const int nx = I.nx, nv = I.nv;
     const int c0 = blockIdx.x * C;
@@ -98,18 +102,31 @@ __global__ __launch_bounds__(NT) void ens_xdrift_kernel(double* __restrict__ poo
     const int j = c0 + c;
     const bool valid = j < nv;
     double* f = pool + I.f_off;
-    const double vel = valid ? tables[I.v_off + j] : 0.0;
-    const double disp = (vel * dt) / I.dx;  // departure(), vlasov1D1V.py:229-238
-    if (g == 0) {
-        disps[c] = disp;
-        line_codes(disp, nx, codes + 2 * c);
+    // departure(), vlasov1D1V.py:229-238: the column's displacement
+    auto disp_of = [&](int cc) {
+        const double v = c0 + cc < nv ? tables[I.v_off + c0 + cc] : 0.0;
+        return (v * dt) / I.dx;
+    };
+    const double disp = disp_of(c);
+    int myq[2];
+    line_codes(disp, nx, myq);
+    // this thread's first rows load while the tables are computed
+    constexpr int PB = kEnsMaxNx / NG < 8 ? kEnsMaxNx / NG : 8;
+    double pre[PB];
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+        const int r = g + k * NG;
+        pre[k] = (valid && r < nx) ? f[static_cast<int64_t>(r) * nv + j] : 0.0;
     }
-    for (int r = g; r < nx; r += NG) bufA[r * C + c] = valid ? f[static_cast<int64_t>(r) * nv + j] : 0.0;
-    __syncthreads();
-    fill_line_tabs(tab, C, disps, codes, NT);
+    fill_line_tabs(tab, C, nx, disp_of, NT);
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+        const int r = g + k * NG;
+        if (r < nx) bufA[r * C + c] = pre[k];
+    }
+    for (int r = g + PB * NG; r < nx; r += NG) bufA[r * C + c] = valid ? f[static_cast<int64_t>(r) * nv + j] : 0.0;
     __syncthreads();
     const double* mytab = tab + c * kEnsTabStride;
-    const int* myq = codes + 2 * c;
     for (int r = g; r < nx; r += NG) bufB[r * C + c] = ens_periodic_sum<C>(bufA, r, disp, mytab, myq, nx, c);
     const double* fin = bufB;  // result of the last pass
     if (PASSES == 2) {
@@ -194,14 +211,27 @@ __global__ __launch_bounds__(kEnsThreads) void ens_kick_kernel(double* __restric
     double* f = pool + I.f_off + static_cast<int64_t>(x0) * nv;
     const int cnt = rows * nv;
     const double* E = Ein + static_cast<int64_t>(blockIdx.y) * kEnsMaxNx + x0;
+    auto disp_of = [&](int rr) { return (E[rr] * dt) / I.dv; };  // departure(), vlasov1D1V.py:229-238
     if (static_cast<int>(threadIdx.x) < rows) {
-        const double disp = (E[threadIdx.x] * dt) / I.dv;  // departure(), vlasov1D1V.py:229-238
+        const double disp = disp_of(threadIdx.x);
         disps[threadIdx.x] = disp;
         line_codes(disp, nv, codes + 2 * threadIdx.x);
     }
-    for (int idx = threadIdx.x; idx < cnt; idx += kEnsThreads) tile[idx] = f[idx];
-    __syncthreads();
-    fill_line_tabs(tab, rows, disps, codes, kEnsThreads);
+    // the first elements of this thread load while the tables are computed
+    constexpr int PB = 8;
+    double pre[PB];
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+        const int idx = static_cast<int>(threadIdx.x) + k * kEnsThreads;
+        pre[k] = idx < cnt ? f[idx] : 0.0;
+    }
+    fill_line_tabs(tab, rows, nv, disp_of, kEnsThreads);
+#pragma unroll
+    for (int k = 0; k < PB; ++k) {
+        const int idx = static_cast<int>(threadIdx.x) + k * kEnsThreads;
+        if (idx < cnt) tile[idx] = pre[k];
+    }
+    for (int idx = static_cast<int>(threadIdx.x) + PB * kEnsThreads; idx < cnt; idx += kEnsThreads) tile[idx] = f[idx];
     __syncthreads();
     for (int idx = threadIdx.x; idx < cnt; idx += kEnsThreads) {
         const int rr = idx / nv;
