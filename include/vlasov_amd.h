@@ -387,7 +387,7 @@ enum {
     VP_KNOB_ENS_XTHREADS = 8,  /* ensemble X drift: 256, 512 or 1024 threads
                                   per workgroup (0 = 512)                      */
     VP_KNOB_SEG_ROWS = 9,      /* generic sweeps of strided lines longer than
-                                  256: 16..256 rows per workgroup segment
+                                  256: 16..128 rows per workgroup segment
                                   (0 = 128, fewer when that gives under
                                   512 workgroups)                              */
     VP_KNOB_SEG_CONTIG = 10,   /* generic sweeps of contiguous lines longer

@@ -611,7 +611,7 @@ static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis,
         constexpr int KC = 256 / static_cast<int>(sizeof(T));
         const int64_t chunks = (inner + KC - 1) / KC;
         int R = knob(VP_KNOB_SEG_ROWS);  // rows per segment (0: 128, fewer for small problems, down to 32)
-        if (R < 16 || R > 256 || (R & (R - 1))) {
+        if (R < 16 || R > 128 || (R & (R - 1))) {  // 128 rows: 49 KB of LDS, under the default 64 KB
             R = 128;
             while (R > 32 && outer * chunks * ((n + R - 1) / R) < 2 * cus) R /= 2;
         }

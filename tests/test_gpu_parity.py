@@ -95,6 +95,35 @@ def test_advect_long_lines(device, shape, axis, spread, periodic):
     np.testing.assert_array_equal(out, ref)
 
 
+def test_segment_lengths_do_not_change_results(device):
+    """The segment-length knobs of the long-line sweeps (strided rows per
+    segment, contiguous indices per segment) choose launch shapes only."""
+    from vlasov_amd import _lib as L
+    v2 = _v2()
+    lib = L.lib()
+    rng = np.random.default_rng(41)
+    f = torch.from_numpy(rng.random((1, 1, 1500, 1200))).to(device)
+    vx = torch.from_numpy(rng.uniform(-30, 30, size=(1, 1, 1200))).to(device)
+    ve = torch.from_numpy(rng.uniform(-8, 8, size=(1, 1, 1500))).to(device)
+    outs = {}
+    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("seg_rows", "seg_contig")}
+    try:
+        for rows, contig in ((0, 0), (16, 256), (32, 512), (64, 2048), (128, 4096)):
+            lib.vp_tune_set(L.KNOBS["seg_rows"], rows)
+            lib.vp_tune_set(L.KNOBS["seg_contig"], contig)
+            a = v2.advect_1d(f, vx, 2, 0.1, 0.25, periodic=True)
+            b = v2.advect_1d(f, ve, 3, 0.1, 0.25, periodic=False)
+            outs[(rows, contig)] = (a, b)
+    finally:
+        for k, v in old.items():
+            lib.vp_tune_set(L.KNOBS[k], v)
+    a0, b0 = outs[(0, 0)]
+    for a, b in outs.values():
+        assert torch.equal(a, a0) and torch.equal(b, b0)
+    ref = vo.advect_axis(f.cpu().numpy(), vx.cpu().numpy(), 2, 0.1, 0.25, periodic=True)
+    np.testing.assert_array_equal(a0.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("periodic", [True, False])
 def test_advect_rows_per_point_velocity(device, periodic):
     v1 = _v1()
