@@ -709,21 +709,26 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
 // h = feq + g in float64 with float64 weights and stores float(K(h) - feq).
 // feq (64 KB at 128 x 128) is read from L2 twice per row: with the new
 // window row and for the output row.
-template <typename S, int NVX, int WPP, int PF, bool NT, bool DEV>
+template <typename S, int NVX, int WPP, int PF, bool NT, bool DEV, int NVY = 128>
 __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict__ fin, S* __restrict__ fout,
                                                              const float* __restrict__ feq,
                                                              const double* __restrict__ Ex,
                                                              const double* __restrict__ Ey, double dt, double dvx,
                                                              double dvy) {
-    constexpr int NVY = 128;
+    // NVY = 128: lane l owns the column pair (2l, 2l + 1), a row is one 16-byte (float64) load per
+    // lane; NVY = 64: lane l owns column l.
+    static_assert(NVY == 128 || NVY == 64, "64 or 128 vy columns");
+    constexpr int CPL = NVY / 64;  // columns per lane
     constexpr int PAD = 16;  // >= 10 on the left, >= 12 on the right (taps of l in [-8, NVY + 8])
     constexpr int RPW = NVX / WPP;
     static_assert(NVX % WPP == 0 && RPW % PF == 0, "rows per wave must be a multiple of PF");
     static_assert(!DEV || sizeof(S) == 4, "deviation storage is float32");
+    static_assert(!DEV || CPL == 2, "deviation storage: 128 vy columns");
     typedef typename std::conditional<DEV, double, S>::type C;  // arithmetic type
-    typedef S V2s __attribute__((ext_vector_type(2)));
-    typedef C V2 __attribute__((ext_vector_type(2)));
-    typedef double V2d __attribute__((ext_vector_type(2)));
+    typedef S V2s_ __attribute__((ext_vector_type(2)));
+    typedef C V2_ __attribute__((ext_vector_type(2)));
+    typedef typename std::conditional<CPL == 2, V2s_, S>::type V2s;  // a lane's part of a row, stored
+    typedef typename std::conditional<CPL == 2, V2_, C>::type V2;    // and in arithmetic
     __shared__ C wx[NVX * 6];
     __shared__ int ilx[NVX];
     __shared__ int irregular[1];
@@ -750,13 +755,13 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
         const int b = lane >> 5, i = lane & 31;
         rowbuf[wv][b][i < PAD ? i : NVY + i] = C(0);
     }
-    C wy[2][6];
-    int ty[2];  // LDS index of tap 0 of each column
+    C wy[CPL][6];
+    int ty[CPL];  // LDS index of tap 0 of each column
     {
         const double ey = Ey[p];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const Departure d = departure(2 * lane + j, ey, dt, dvy);
+        for (int j = 0; j < CPL; ++j) {
+            const Departure d = departure(CPL * lane + j, ey, dt, dvy);
             double w[6];
             lagrange5_weights<double>(d.alpha, w);
             round_weights<C>(w, wy[j]);
@@ -764,7 +769,8 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             ty[j] = PAD + l - 2;
         }
     }
-    const bool pair = __ballot(ty[1] != ty[0] + 1) == 0;  // wave-uniform
+    bool pair = false;
+    if constexpr (CPL == 2) pair = __ballot(ty[1] != ty[0] + 1) == 0;  // wave-uniform
     __syncthreads();
     const int s0 = ilx[0];
     for (int r = t; r < NVX; r += 64 * WPP)
@@ -772,8 +778,8 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     __syncthreads();
     const bool regular = irregular[0] == 0;  // uniform over the workgroup
 
-    const S* src = fin + p * NVX * NVY + 2 * lane;
-    S* dst = fout + p * NVX * NVY + 2 * lane;
+    const S* src = fin + p * NVX * NVY + CPL * lane;
+    S* dst = fout + p * NVX * NVY + CPL * lane;
     const float* eq = DEV ? feq + 2 * lane : nullptr;
     typedef float V2f __attribute__((ext_vector_type(2)));
     // A row as loaded (g, and feq for DEV) and as used (h = feq + g, or f).
@@ -798,47 +804,64 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
         }
         return v;
     };
+    (void)val;
     auto clampr = [](int j) { return j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j); };
     auto maskr = [](int j) { return (j >= 0 && j < NVX) ? C(1) : C(0); };
     // Vy kick of row r from the Vx results ox of the lane's two columns;
     // eo: feq of row r at those columns (DEV).
     auto vy_row = [&](int r, C ox0, C ox1, V2f eo) {
         C* rb = rowbuf[wv][r & 1];
-        V2 o;
-        o.x = ox0;
-        o.y = ox1;
-        *reinterpret_cast<V2*>(&rb[PAD + 2 * lane]) = o;
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        C a[6], b[6];
-        if (pair) {
-            C v[7];
+        if constexpr (CPL == 1) {
+            rb[PAD + lane] = ox0;
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            C a[6];
 #pragma unroll
-            for (int i = 0; i < 7; ++i) v[i] = rb[ty[0] + i];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                a[k] = v[k];
-                b[k] = v[k + 1];
-            }
+            for (int k = 0; k < 6; ++k) a[k] = rb[ty[0] + k];
+            const S res = tap_sum<C>(wy[0], a);
+            S* out = dst + r * NVY;
+            if (NT)
+                __builtin_nontemporal_store(res, out);
+            else
+                *out = res;
+            (void)ox1;
+            (void)eo;
         } else {
+            V2_ o;
+            o.x = ox0;
+            o.y = ox1;
+            *reinterpret_cast<V2_*>(&rb[PAD + 2 * lane]) = o;
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            C a[6], b[6];
+            if (pair) {
+                C v[7];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                a[k] = rb[ty[0] + k];
-                b[k] = rb[ty[1] + k];
+                for (int i = 0; i < 7; ++i) v[i] = rb[ty[0] + i];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    a[k] = v[k];
+                    b[k] = v[k + 1];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    a[k] = rb[ty[0] + k];
+                    b[k] = rb[ty[1] + k];
+                }
             }
+            V2s res;
+            if constexpr (DEV) {
+                res.x = static_cast<float>(tap_sum<double>(wy[0], a) - static_cast<double>(eo.x));
+                res.y = static_cast<float>(tap_sum<double>(wy[1], b) - static_cast<double>(eo.y));
+            } else {
+                res.x = tap_sum<C>(wy[0], a);
+                res.y = tap_sum<C>(wy[1], b);
+            }
+            V2s* out = reinterpret_cast<V2s*>(dst + r * NVY);
+            if (NT)
+                __builtin_nontemporal_store(res, out);
+            else
+                *out = res;
         }
-        V2s res;
-        if constexpr (DEV) {
-            res.x = static_cast<float>(tap_sum<double>(wy[0], a) - static_cast<double>(eo.x));
-            res.y = static_cast<float>(tap_sum<double>(wy[1], b) - static_cast<double>(eo.y));
-        } else {
-            res.x = tap_sum<C>(wy[0], a);
-            res.y = tap_sum<C>(wy[1], b);
-        }
-        V2s* out = reinterpret_cast<V2s*>(dst + r * NVY);
-        if (NT)
-            __builtin_nontemporal_store(res, out);
-        else
-            *out = res;
     };
     // feq of output row r (DEV; an on-chip read: the 64 KB table stays in L2)
     auto eq_row = [&](int r) -> V2f {
@@ -851,11 +874,18 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             w[k] = wx[r * 6 + k];
-            a[k] = v[k].x;
-            b[k] = v[k].y;
+            if constexpr (CPL == 2) {
+                a[k] = v[k].x;
+                b[k] = v[k].y;
+            } else {
+                a[k] = v[k];
+            }
         }
         ox0 = tap_sum<C>(w, a);
-        ox1 = tap_sum<C>(w, b);
+        if constexpr (CPL == 2)
+            ox1 = tap_sum<C>(w, b);
+        else
+            ox1 = C(0);
     };
 
     const int r_lo = wv * RPW;
@@ -1200,7 +1230,7 @@ static int launch_kick_stream(const S* fin, S* fout, int nplanes, const double* 
 // Row-split kick, 128 vy columns, WPP waves per plane, pf8: 8 rows
 // prefetched (else 4), at most the rows per wave; feq != nullptr selects
 // deviation storage (float only).
-template <typename S, int NVX, int WPP>
+template <typename S, int NVX, int WPP, int NVY>
 static int launch_kick_rows(bool pf8, const S* fin, S* fout, const float* feq, int nplanes, const double* Ex,
                             const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
     constexpr int RPW = NVX / WPP;
@@ -1209,12 +1239,12 @@ static int launch_kick_rows(bool pf8, const S* fin, S* fout, const float* feq, i
     constexpr bool F32 = sizeof(S) == 4;
     const bool nt = (nt_mask() & 4) != 0;
 #define VP_K(PF, NT, DEV)                                                                                     \
-    hipLaunchKernelGGL((kick_rows_kernel<S, NVX, WPP, PF, NT, DEV>), dim3(nplanes), dim3(64 * WPP), 0, s, fin, \
-                       fout, feq, Ex, Ey, dt, dvx, dvy)
+    hipLaunchKernelGGL((kick_rows_kernel<S, NVX, WPP, PF, NT, DEV, NVY>), dim3(nplanes), dim3(64 * WPP), 0, s, \
+                       fin, fout, feq, Ex, Ey, dt, dvx, dvy)
 #define VP_K2(PF, DEV)                      \
     if (nt) VP_K(PF, true, DEV); else VP_K(PF, false, DEV)
     if (feq) {
-        if constexpr (F32) {
+        if constexpr (F32 && NVY == 128) {
             if (pf8) { VP_K2(PF8, true); } else { VP_K2(PF4, true); }
         } else {
             return VP_ERR_UNSUPPORTED;
@@ -1228,20 +1258,20 @@ static int launch_kick_rows(bool pf8, const S* fin, S* fout, const float* feq, i
     return VP_OK;
 }
 
-// Kick kernel choice at 128 vy columns: the kick-kernel knob 3, 4, 5, 6 picks
-// the row-split kernel with 2, 4, 8, 16 waves per plane, 2 the two-wave
-// row-streaming kernel; the kick-prefetch knob 1 or 3 picks 4 or 8 rows
-// prefetched.  Automatic (knobs 0 and 2): 8 waves per plane, 4 rows
-// prefetched for double and 8 for float (measured fastest at 128^4).
-// Deviation storage always takes a row-split kernel: automatic = 2 waves
-// per plane and 4 rows prefetched (float64 arithmetic: ~125 registers).
-template <typename S, int NVX>
+// Kick kernel choice at 128 vy columns (and 64 for float64): the kick-kernel
+// knob 3, 4, 5, 6 picks the row-split kernel with 2, 4, 8, 16 waves per
+// plane, 2 the two-wave row-streaming kernel; the kick-prefetch knob 1 or 3
+// picks 4 or 8 rows prefetched.  Automatic (knobs 0 and 2): 8 waves per
+// plane, 4 rows prefetched for double and 8 for float (measured fastest at
+// 128^4).  Deviation storage always takes a row-split kernel: automatic = 2
+// waves per plane and 4 rows prefetched (float64 arithmetic: ~125 registers).
+template <typename S, int NVX, int NVY = 128>
 static int kick_rows_dispatch(int kk, int pfk, const S* fin, S* fout, const float* feq, int nplanes,
                               const double* Ex, const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
     if (feq && (kk < 3 || kk > 6)) kk = 3;
     if (kk == 0) kk = 5;
     const bool pf8 = pfk == 3 || (pfk != 1 && sizeof(S) == 4 && !feq);
-#define VP_R(W) return launch_kick_rows<S, NVX, W>(pf8, fin, fout, feq, nplanes, Ex, Ey, dt, dvx, dvy, s)
+#define VP_R(W) return launch_kick_rows<S, NVX, W, NVY>(pf8, fin, fout, feq, nplanes, Ex, Ey, dt, dvx, dvy, s)
     if (kk == 3) VP_R(2);
     if (kk == 4) VP_R(4);
     if (kk == 6 && NVX >= 64) VP_R(16);
@@ -1255,13 +1285,19 @@ static int kick_stream(const S* fin, S* fout, const int64_t shape[4], const doub
                        double dvx, double dvy, hipStream_t s) {
     const int64_t nvx = shape[2], nvy = shape[3];
     const int nplanes = static_cast<int>(shape[0] * shape[1]);
-    if (nvy == 128) {
+    if (nvy == 128 || (nvy == 64 && sizeof(S) == 8)) {
         const int kk = knob(VP_KNOB_KICK_KERNEL), pfk = knob(VP_KNOB_KICK_PREFETCH);
         int rc = VP_ERR_UNSUPPORTED;
-#define VP_RD(N) rc = kick_rows_dispatch<S, N>(kk, pfk, fin, fout, nullptr, nplanes, Ex, Ey, dt, dvx, dvy, s)
-        if (nvx == 128) VP_RD(128);
-        if (nvx == 64) VP_RD(64);
-        if (nvx == 32) VP_RD(32);
+#define VP_RD(N, M) rc = kick_rows_dispatch<S, N, M>(kk, pfk, fin, fout, nullptr, nplanes, Ex, Ey, dt, dvx, dvy, s)
+        if (nvy == 128) {
+            if (nvx == 128) VP_RD(128, 128);
+            if (nvx == 64) VP_RD(64, 128);
+            if (nvx == 32) VP_RD(32, 128);
+        } else {
+            if (nvx == 128) VP_RD(128, 64);
+            if (nvx == 64) VP_RD(64, 64);
+            if (nvx == 32) VP_RD(32, 64);
+        }
 #undef VP_RD
         if (rc != VP_ERR_UNSUPPORTED) return rc;
     }

@@ -153,11 +153,12 @@ def test_vlasov2d_driver_fused_landau(device, tmp_path):
     assert st.step == 20 and (tmp_path / "field_000010.npz").exists() and (tmp_path / "field_000020.npz").exists()
 
 
-@pytest.mark.parametrize("shape", [(8, 4, 64, 128), (4, 4, 128, 128), (4, 8, 32, 128), (8, 8, 128, 64)])
+@pytest.mark.parametrize("shape", [(8, 4, 64, 128), (4, 4, 128, 128), (4, 8, 32, 128), (8, 8, 128, 64),
+                                   (8, 4, 64, 64), (4, 4, 32, 64)])
 def test_kick_stream_and_plane_kernels_agree_on_edge_fields(device, shape):
-    """Out-of-place kicks (row-split kernel at 128 vy columns with 2, 4, 8 and
-    16 waves per plane and 4 or 8 prefetched rows, two-wave row-streaming
-    kernel), the in-place whole-plane
+    """Out-of-place kicks (row-split kernel at 128 or 64 vy columns with 2, 4,
+    8 and 16 waves per plane and 4 or 8 prefetched rows, two-wave
+    row-streaming kernel), the in-place whole-plane
     LDS kick and the oracle agree bit for bit on fields that exercise every
     tap path: E = 0, tiny E (rounding makes floor(r - d) - r non-constant),
     multi-cell and fully out-of-domain shifts, both signs."""
