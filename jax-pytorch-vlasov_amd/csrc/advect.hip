@@ -421,11 +421,11 @@ __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ f
     if (!cval) return;
     if (FORM == kDeparture) {
         const T* mytab = tab + c * TS;
+        LineWeights<T> lw;
         for (int r = r0 + g; r < r1; r += NG) {
             const double dep = static_cast<double>(r) - ls.disp;
             const double fl = floor(dep);
-            T w[6];
-            departure_weights<T>(dep, fl, mytab, q0, nq, w);
+            const T* w = lw.get(dep, fl, mytab, q0, nq);
             int j0 = static_cast<int>(fl) - 2 - lo;  // staged row of tap 0, in [1, cnt - 7] for finite disp
             j0 = j0 < 0 ? 0 : (j0 > cnt - 6 ? cnt - 6 : j0);
             T v[6];
@@ -510,11 +510,11 @@ __global__ __launch_bounds__(256) void advect_cseg_kernel(const T* __restrict__ 
         tile[idx] = staged(idx);
     __syncthreads();
     if (FORM == kDeparture) {
+        LineWeights<T> lw;
         for (int r = r0 + static_cast<int>(threadIdx.x); r < r1; r += blockDim.x) {
             const double dep = static_cast<double>(r) - ls.disp;
             const double fl = floor(dep);
-            T w[6];
-            departure_weights<T>(dep, fl, tab, q0, nq, w);
+            const T* w = lw.get(dep, fl, tab, q0, nq);
             int j0 = static_cast<int>(fl) - 2 - lo;  // staged index of tap 0, in [1, cnt - 7]
             j0 = j0 < 0 ? 0 : (j0 > cnt - 6 ? cnt - 6 : j0);
             T v[6];

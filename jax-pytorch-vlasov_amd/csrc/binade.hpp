@@ -98,6 +98,28 @@ __device__ __forceinline__ void departure_weights(double dep, double fl, const T
     }
 }
 
+// departure_weights for a thread walking along one line: the weights of the
+// last table code stay in registers and are read again only when the code
+// changes (departure points without a code compute theirs).
+template <typename T>
+struct LineWeights {
+    T w[6];
+    int cur = kNoCode;
+    __device__ __forceinline__ const T* get(double dep, double fl, const T* tab, int q0, int nq) {
+        const int code = binade_code(dep);
+        const int slot = code - q0;
+        if (code == kNoCode || slot < 0 || slot >= nq) {
+            weights_as<T>(dep - fl, w);
+            cur = kNoCode;
+        } else if (code != cur) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[k] = tab[slot * 6 + k];
+            cur = code;
+        }
+        return w;
+    }
+};
+
 // Binade codes a segment of departure points [dlo, dhi] reaches, clipped to
 // the coded range: [q0, q0 + nq).
 __device__ __forceinline__ void segment_codes(double dlo, double dhi, int& q0, int& nq) {
