@@ -5,7 +5,7 @@ with no fused multiply-add, so they must agree to <= 1e-14 relative (and are
 expected to be bit-identical, checked separately); reductions and FFTs
 round differently (compensated sums, exact-twiddle DFTs), so field results
 agree to <= 1e-13 of their max magnitude and multi-step trajectories to
-<= 1e-12 relative (the north-star parity bar).
+<= 1e-12 relative (the parity bar for the float64 path).
 """
 import math
 
