@@ -34,7 +34,7 @@ enum {
     VP_ERR_ARG = -1,        /* null pointer, bad axis / flag                 */
     VP_ERR_SHAPE = -2,      /* size out of the supported range              */
     VP_ERR_WORKSPACE = -3,  /* ws too small (see the *_workspace query)     */
-    VP_ERR_UNSUPPORTED = -4 /* shape/variant this build has no kernel for   */
+    VP_ERR_UNSUPPORTED = -4 /* shape/variant this library has no kernel for */
 };
 
 const char* vp_error_string(int code);

@@ -32,7 +32,7 @@ extern "C" const char* vp_error_string(int code) {
         case VP_ERR_ARG: return "invalid argument (null pointer, axis or flag)";
         case VP_ERR_SHAPE: return "array size outside the supported range";
         case VP_ERR_WORKSPACE: return "workspace too small";
-        case VP_ERR_UNSUPPORTED: return "shape or variant not supported by this build";
+        case VP_ERR_UNSUPPORTED: return "shape or variant not supported by this library";
         default: break;
     }
     if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));

@@ -428,7 +428,7 @@ def vlasov_step_1d_reflected(f, dt, v, inv_kx, inv_kx_sq, dx, dv, physics_mode=T
 
 
 # --------------------------------------------------------------------------
-# Diagnostics (no reference function; defined by this build)
+# Diagnostics (no reference function; defined by this library)
 # --------------------------------------------------------------------------
 
 def field_energy_1d(E, dx):
