@@ -165,6 +165,38 @@ def test_fp32_xdrift_column_pairs_equal_single_columns(device, shape, passes):
     assert not torch.equal(outs[0], f)
 
 
+@pytest.mark.parametrize("shape,axis,periodic", [((1, 1, 1000, 300), 2, True), ((2, 1, 40, 2000), 3, False),
+                                                 ((600, 2, 3, 70), 0, True)])
+def test_fp32_long_line_sweeps(device, shape, axis, periodic):
+    """Float32 sweeps of lines longer than 256 (segmented kernels): within
+    float32 rounding of the float64 oracle on the same (rounded) input, and
+    the same bits for every segment length (which also moves segments to the
+    global-memory fallback)."""
+    v2, runner, L = _mods()
+    lib = L.lib()
+    rng = np.random.default_rng(sum(shape))
+    f32 = torch.from_numpy(rng.random(shape)).float().to(device)
+    batch = tuple(s for i, s in enumerate(shape) if i != axis)
+    vel = torch.from_numpy(rng.uniform(-1, 1, size=batch) * 40.0).to(device)
+    dt, dx = 0.1, 0.25
+    outs = []
+    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("seg_rows", "seg_contig")}
+    try:
+        for rows, contig in ((0, 0), (16, 256), (128, 4096)):
+            lib.vp_tune_set(L.KNOBS["seg_rows"], rows)
+            lib.vp_tune_set(L.KNOBS["seg_contig"], contig)
+            outs.append(v2.advect_1d(f32, vel, axis, dt, dx, periodic=periodic))
+    finally:
+        for k, v in old.items():
+            lib.vp_tune_set(L.KNOBS[k], v)
+    assert outs[0].dtype == torch.float32
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    ref = vo.advect_axis(f32.double().cpu().numpy(), vel.cpu().numpy(), axis, dt, dx, periodic=periodic)
+    err = np.max(np.abs(outs[0].double().cpu().numpy() - ref)) / np.max(np.abs(ref))
+    assert err <= 2e-6, err
+
+
 def test_fp32_rejects_in_place_kick_and_small_shapes(device):
     v2, runner, L = _mods()
     lib = L.lib()
