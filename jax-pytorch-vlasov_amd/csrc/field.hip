@@ -135,18 +135,22 @@ __global__ __launch_bounds__(256) void p1d_inv_kernel(const double2* __restrict_
 // Z_{nx-k} = conj(E_k) + i conj(P_k), and only the real parts of the DC and
 // Nyquist bins, as irfft reads them), whose inverse FFT is e + i p.  Same
 // transform as the DFT kernels, O(nx log nx) instead of O(nx^2); agrees to
-// rounding.  Twiddles exp(-2 pi i j / nx), j < nx/2, from sincospi.
+// rounding.  Twiddles from sincospi, one contiguous table per stage: the
+// stage of half-length h reads exp(-i pi p / h), p < h, at tw[h - 1 + p]
+// (n - 1 entries), so the lanes of a butterfly stage read consecutive
+// entries instead of one entry every n / (2 h) (which put every lane of a
+// wave in the same LDS bank).
 constexpr int kFftMaxN = 4096;
 
 __device__ __forceinline__ void fft_radix2(double2* a, const double2* tw, int nx, bool inverse) {
     const int half_n = nx >> 1;
     for (int half = 1; half < nx; half <<= 1) {
-        const int step = half_n / half;  // twiddle stride of this stage
+        const double2* tws = tw + (half - 1);  // this stage's table
         for (int t = threadIdx.x; t < half_n; t += blockDim.x) {
             const int pos = t & (half - 1);
             const int i = ((t - pos) << 1) + pos;  // group * 2 half + pos
             const int j = i + half;
-            double2 w = tw[pos * step];
+            double2 w = tws[pos];
             if (inverse) w.y = -w.y;
             const double2 u = a[i];
             const double2 b = a[j];
@@ -164,12 +168,14 @@ __global__ __launch_bounds__(1024) void poisson1d_fft_kernel(const double* __res
                                                              double* __restrict__ E, double* __restrict__ phi) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double2* a = reinterpret_cast<double2*>(smem_raw);  // [nx]
-    double2* tw = a + nx;                                 // [nx / 2]
+    double2* tw = a + nx;                                 // [nx - 1]: per-stage tables
     const int half_n = nx >> 1;
-    for (int j = threadIdx.x; j < half_n; j += blockDim.x) {
+    for (int e = threadIdx.x; e < nx - 1; e += blockDim.x) {
+        const int h = 1 << (31 - __clz(e + 1));  // stage of entry e: h - 1 <= e < 2 h - 1
+        const int pos = e - (h - 1);
         double sn, cs;
-        sincospi(2.0 * static_cast<double>(j) / static_cast<double>(nx), &sn, &cs);
-        tw[j] = make_double2(cs, -sn);
+        sincospi(static_cast<double>(pos) / static_cast<double>(h), &sn, &cs);  // exp(-i pi pos / h)
+        tw[e] = make_double2(cs, -sn);
     }
     for (int x = threadIdx.x; x < nx; x += blockDim.x)
         a[__brev(static_cast<unsigned>(x)) >> (32 - lg)] = make_double2(rho[x], 0.0);
@@ -377,10 +383,10 @@ int poisson1d(const double* rho, int64_t nx, const double* inv_kx, const double*
         if (!attr) {
             VP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&poisson1d_fft_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           static_cast<int>(kFftMaxN / 2 * 3 * sizeof(double2))));
+                                           static_cast<int>(2 * kFftMaxN * sizeof(double2))));
             attr = true;
         }
-        const size_t lds = static_cast<size_t>(n / 2 * 3) * sizeof(double2);
+        const size_t lds = static_cast<size_t>(2 * n - 1) * sizeof(double2);
         hipLaunchKernelGGL(poisson1d_fft_kernel, dim3(1), dim3(1024), lds, s, rho, n, lg, inv_kx, inv_kx_sq, E, phi);
         VP_CHECK_LAUNCH();
         return VP_OK;
