@@ -388,7 +388,7 @@ enum {
                                   per workgroup (0 = 512)                      */
     VP_KNOB_SEG_ROWS = 9,      /* generic sweeps of strided lines longer than
                                   256: 16..128 rows per workgroup segment
-                                  (0 = 128, fewer when that gives under
+                                  (0 = 112, or 64 / 32 when that gives under
                                   512 workgroups)                              */
     VP_KNOB_SEG_CONTIG = 10,   /* generic sweeps of contiguous lines longer
                                   than 256: 256..4096 indices per workgroup

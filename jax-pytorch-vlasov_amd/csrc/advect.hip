@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
 // lowest departure point up (an odd column stride puts the columns' tables
 // in different LDS banks).  A segment whose rows would not fit reads its
 // taps from global memory.
-constexpr int kSegBinades = 8;
+constexpr int kSegBinades = 4;  // with 112-row segments: 38 KB of LDS, four workgroups per CU
 constexpr int kSegPrefetch = 8;  // staged elements per thread loaded before the tables are filled
 
 template <typename T, bool PERIODIC, int FORM>
@@ -610,12 +610,17 @@ static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis,
     if (inner > 1 && seg_ok && inner >= 256 / static_cast<int64_t>(sizeof(T))) {
         constexpr int KC = 256 / static_cast<int>(sizeof(T));
         const int64_t chunks = (inner + KC - 1) / KC;
-        int R = knob(VP_KNOB_SEG_ROWS);  // rows per segment (0: 128, fewer for small problems, down to 32)
-        if (R < 16 || R > 128 || (R & (R - 1))) {  // 128 rows: 49 KB of LDS, under the default 64 KB
-            R = 128;
-            while (R > 32 && outer * chunks * ((n + R - 1) / R) < 2 * cus) R /= 2;
+        // rows per segment: 112 (its tile and tables fit four workgroups per CU in LDS) when that
+        // gives at least two workgroups per CU, else 64 or 32
+        int R = knob(VP_KNOB_SEG_ROWS);
+        if (R < 16 || R > 128) {  // 128 rows: 42 KB of LDS, under the default 64 KB
+            R = 112;
+            if (outer * chunks * ((n + R - 1) / R) < 2 * cus) {
+                R = 64;
+                while (R > 32 && outer * chunks * ((n + R - 1) / R) < 2 * cus) R /= 2;
+            }
         }
-        const int rows_cap = R + 16;
+        const int rows_cap = R + 12;  // the segment, a displacement spread of up to 4 rows and the stencil
         const int n_segs = (n + R - 1) / R;
         const int64_t blocks = outer * chunks * n_segs;
         VP_RETURN_IF(blocks > 0x7fffffffLL, VP_ERR_SHAPE);
