@@ -247,6 +247,17 @@ __device__ __forceinline__ f32x2 tap_sum_e(const float (&w)[6], const f32x2 (&v)
 // (A + 256 k) mod 256 n: two integer operations per tap from one register per
 // row, instead of six precomputed offsets (which the compiler would otherwise
 // hoist out of the tile loop: 6 registers per row).
+// The same six taps when none of them wraps around the tile (tap-0 row at
+// most n - 6): one base address, the taps at immediate offsets.
+template <typename E, typename S>
+__device__ __forceinline__ E ring_tap_sum_direct(const E* tile, uint32_t A, const S (&w)[6]) {
+    const char* p = reinterpret_cast<const char*>(tile) + A;
+    E v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = *reinterpret_cast<const E*>(p + 256 * k);
+    return tap_sum_e(w, v);
+}
+
 template <typename E, typename S, int N>
 __device__ __forceinline__ E ring_tap_sum(const E* tile, uint32_t A, const S (&w)[6]) {
     const char* base = reinterpret_cast<const char*>(tile);
@@ -332,6 +343,15 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
         for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w[i][k]));
         asm volatile("" ::"v"(A[i]));
     }
+    // float32 storage: rows whose taps wrap for no lane of the wave (all but
+    // a few next to the line ends) read them at immediate offsets, dropping
+    // two integer operations per tap; float32 interpolation costs as many
+    // VALU cycles per element as float64 (no packed math), so the float
+    // drifts are the ones this saves time in (2 %; float64: no change).
+    constexpr bool DIRECT = sizeof(S) == 4;
+    bool direct[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) direct[i] = DIRECT && __all((A[i] >> 8) <= static_cast<uint32_t>(n - 6));
     const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / nsplit);
     const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / nsplit);
     if (l0 >= l1) return;  // uniform over the workgroup
@@ -384,13 +404,16 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
         if (PASSES == 2) {
             // first pass into `mid` (free since the barrier above), second from it
 #pragma unroll
-            for (int ii = 0; ii < R; ++ii) mid[(g + KG * ii) * KC + c] = ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
+            for (int ii = 0; ii < R; ++ii)
+                mid[(g + KG * ii) * KC + c] = direct[ii] ? ring_tap_sum_direct<E, S>(tile, A[ii], w[ii])
+                                                         : ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
             __syncthreads();
             tile = mid;
         }
 #pragma unroll
         for (int ii = 0; ii < R; ++ii) {
-            const E o = ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
+            const E o = direct[ii] ? ring_tap_sum_direct<E, S>(tile, A[ii], w[ii])
+                                   : ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
             store_f<NT>(&fout[base + (g + KG * ii) * row_stride + l * loop_stride], o);
             if constexpr (RHO) {
                 if (COMP)
