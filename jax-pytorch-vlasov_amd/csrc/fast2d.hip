@@ -1034,10 +1034,14 @@ static inline size_t a256(size_t b) { return (b + 255) & ~static_cast<size_t>(25
 static int y_split(int64_t nvx, int elem_bytes) {
     int S = knob(VP_KNOB_Y_SPLIT);
     if (S <= 0) {
-        // Measured at 128^4: float64 best at 12 splits (10-11 tiles per
+        // Measured: at 128^4 float64 best at 12 splits (10-11 tiles per
         // workgroup; 1, 8, 16, 32 slower), float32 storage at 10 (6, 8, 12, 16
-        // slower).
-        S = elem_bytes == 8 ? static_cast<int>((3 * nvx + 16) / 32) : static_cast<int>((5 * nvx + 32) / 64);
+        // slower); at 64^4 both at 12 (2-8, 16, 24, 32 slower; float32 Y
+        // drifts 9 % faster than at 5).
+        if (nvx == 64)
+            S = 12;
+        else
+            S = elem_bytes == 8 ? static_cast<int>((3 * nvx + 16) / 32) : static_cast<int>((5 * nvx + 32) / 64);
         if (S < 1) S = 1;
     }
     if (S > nvx) S = static_cast<int>(nvx);
