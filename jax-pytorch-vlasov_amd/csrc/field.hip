@@ -128,8 +128,8 @@ __global__ __launch_bounds__(256) void p1d_inv_kernel(const double2* __restrict_
 }
 
 // 1D, power-of-two nx (8 to 4096): one workgroup, radix-2 FFTs in LDS.
-// Forward: the complex FFT of rho (bit-reversed load, decimation in time)
-// gives X_k; E_k = -i X_k inv_kx[k], P_k = X_k inv_kx_sq[k] for
+// Forward: the complex FFT of rho (decimation in frequency, bit-reversed
+// output) gives X_k; E_k = -i X_k inv_kx[k], P_k = X_k inv_kx_sq[k] for
 // k <= nx/2.  Both inverses at once: E and phi are real, so their
 // Hermitian spectra combine into Z = E + i P (Z_k = E_k + i P_k,
 // Z_{nx-k} = conj(E_k) + i conj(P_k), and only the real parts of the DC and
@@ -162,6 +162,28 @@ __device__ __forceinline__ void fft_radix2(double2* a, const double2* tw, int nx
     }
 }
 
+// Decimation in frequency: natural-order input, bit-reversed output (the
+// order the decimation-in-time inverse reads), so neither transform needs a
+// permutation pass (scattered LDS accesses).
+__device__ __forceinline__ void fft_radix2_dif(double2* a, const double2* tw, int nx) {
+    const int half_n = nx >> 1;
+    for (int half = half_n; half >= 1; half >>= 1) {
+        const double2* tws = tw + (half - 1);
+        for (int t = threadIdx.x; t < half_n; t += blockDim.x) {
+            const int pos = t & (half - 1);
+            const int i = ((t - pos) << 1) + pos;
+            const int j = i + half;
+            const double2 w = tws[pos];
+            const double2 u = a[i];
+            const double2 b = a[j];
+            const double2 d = make_double2(u.x - b.x, u.y - b.y);
+            a[i] = make_double2(u.x + b.x, u.y + b.y);
+            a[j] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(1024) void poisson1d_fft_kernel(const double* __restrict__ rho, int nx, int lg,
                                                              const double* __restrict__ inv_kx,
                                                              const double* __restrict__ inv_kx_sq,
@@ -177,34 +199,25 @@ __global__ __launch_bounds__(1024) void poisson1d_fft_kernel(const double* __res
         sincospi(static_cast<double>(pos) / static_cast<double>(h), &sn, &cs);  // exp(-i pi pos / h)
         tw[e] = make_double2(cs, -sn);
     }
-    for (int x = threadIdx.x; x < nx; x += blockDim.x)
-        a[__brev(static_cast<unsigned>(x)) >> (32 - lg)] = make_double2(rho[x], 0.0);
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) a[x] = make_double2(rho[x], 0.0);
     __syncthreads();
-    fft_radix2(a, tw, nx, false);
+    fft_radix2_dif(a, tw, nx);  // X_k at a[brev(k)]
+    auto br = [lg](int k) { return static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - lg)); };
     // each thread reads bins k <= nx/2 it alone writes; bins above nx/2 are not read
     for (int k = threadIdx.x; k <= half_n; k += blockDim.x) {
-        const double2 X = a[k];
+        const double2 X = a[br(k)];
         const double ik = inv_kx[k], ik2 = inv_kx_sq[k];
         const double er = X.y * ik, ei = -X.x * ik;
         const double pr = X.x * ik2, pi = X.y * ik2;
         if (k == 0 || k == half_n) {
-            a[k] = make_double2(er, pr);
+            a[br(k)] = make_double2(er, pr);
         } else {
-            a[k] = make_double2(er - pi, ei + pr);
-            a[nx - k] = make_double2(er + pi, pr - ei);
+            a[br(k)] = make_double2(er - pi, ei + pr);
+            a[br(nx - k)] = make_double2(er + pi, pr - ei);
         }
     }
     __syncthreads();
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-        const int y = static_cast<int>(__brev(static_cast<unsigned>(x)) >> (32 - lg));
-        if (x < y) {
-            const double2 t = a[x];
-            a[x] = a[y];
-            a[y] = t;
-        }
-    }
-    __syncthreads();
-    fft_radix2(a, tw, nx, true);
+    fft_radix2(a, tw, nx, true);  // bit-reversed spectrum in, natural order out
     const double inv_n = 1.0 / static_cast<double>(nx);  // exact for a power of two
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
         E[x] = a[x].x * inv_n;
