@@ -313,7 +313,8 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
 // in different LDS banks).  A segment whose rows would not fit reads its
 // taps from global memory.
 constexpr int kSegBinades = 4;  // with 112-row segments: 38 KB of LDS, four workgroups per CU
-constexpr int kSegPrefetch = 8;  // staged elements per thread loaded before the tables are filled
+constexpr int kSegMaxRows = 128;  // rows per segment at most (its staged rows: kSegMaxRows + 12)
+constexpr int kSegPrefetch = 8;  // contiguous segments: staged elements per thread loaded before the table fill
 
 template <typename T, bool PERIODIC, int FORM>
 __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ fin, T* __restrict__ fout, int n,
@@ -322,6 +323,9 @@ __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ f
     constexpr int KC = 256 / static_cast<int>(sizeof(T));
     constexpr int NG = 256 / KC;
     constexpr int TS = FORM == kDeparture ? kSegBinades * 6 + 1 : 0;
+    // staged rows per thread loaded at once: all of them for 8-byte T (18),
+    // 18 of up to 35 for 4-byte T (more would cut the waves per SIMD)
+    constexpr int PF = (kSegMaxRows + 12 + NG - 1) / NG < 18 ? (kSegMaxRows + 12 + NG - 1) / NG : 18;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* tab = reinterpret_cast<T*>(smem_raw);  // [KC][TS]
     T* tile = tab + KC * TS;                    // [rows_cap][KC]
@@ -389,11 +393,11 @@ __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ f
         }
         return v;
     };
-    // the first kSegPrefetch staged rows of this thread are loading while the
-    // weight tables are computed
-    T pre[kSegPrefetch];
+    // the first PF staged rows of this thread are loading at once, while the
+    // weight tables are computed (a load per row in turn leaves one load in flight)
+    T pre[PF];
 #pragma unroll
-    for (int k = 0; k < kSegPrefetch; ++k) {
+    for (int k = 0; k < PF; ++k) {
         const int rr = g + k * NG;
         pre[k] = rr < cnt ? staged(rr) : T(0);
     }
@@ -412,11 +416,11 @@ __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ f
         }
     }
 #pragma unroll
-    for (int k = 0; k < kSegPrefetch; ++k) {
+    for (int k = 0; k < PF; ++k) {
         const int rr = g + k * NG;
         if (rr < cnt) tile[rr * KC + c] = pre[k];
     }
-    for (int rr = g + kSegPrefetch * NG; rr < cnt; rr += NG) tile[rr * KC + c] = staged(rr);
+    for (int rr = g + PF * NG; rr < cnt; rr += NG) tile[rr * KC + c] = staged(rr);
     __syncthreads();
     if (!cval) return;
     if (FORM == kDeparture) {
@@ -613,7 +617,7 @@ static int launch_sweep(const T* fin, T* fout, const int64_t shape[4], int axis,
         // rows per segment: 112 (its tile and tables fit four workgroups per CU in LDS) when that
         // gives at least two workgroups per CU, else 64 or 32
         int R = knob(VP_KNOB_SEG_ROWS);
-        if (R < 16 || R > 128) {  // 128 rows: 42 KB of LDS, under the default 64 KB
+        if (R < 16 || R > kSegMaxRows) {  // 128 rows: 42 KB of LDS, under the default 64 KB
             R = 112;
             if (outer * chunks * ((n + R - 1) / R) < 2 * cus) {
                 R = 64;
