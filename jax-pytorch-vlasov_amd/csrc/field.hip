@@ -127,7 +127,8 @@ __global__ __launch_bounds__(256) void p1d_inv_kernel(const double2* __restrict_
     phi[x] = sp * inv_n;
 }
 
-// 1D, power-of-two nx (8 to 4096): one workgroup, radix-2 FFTs in LDS.
+// 1D, power-of-two nx (8 to 4096): one workgroup, radix-2 FFTs in LDS (up
+// to three stages per pass through LDS).
 // Forward: the complex FFT of rho (decimation in frequency, bit-reversed
 // output) gives X_k; E_k = -i X_k inv_kx[k], P_k = X_k inv_kx_sq[k] for
 // k <= nx/2.  Both inverses at once: E and phi are real, so their
@@ -142,73 +143,157 @@ __global__ __launch_bounds__(256) void p1d_inv_kernel(const double2* __restrict_
 // wave in the same LDS bank).
 constexpr int kFftMaxN = 4096;
 
-__device__ __forceinline__ void fft_radix2(double2* a, const double2* tw, int nx, bool inverse) {
-    const int half_n = nx >> 1;
-    for (int half = 1; half < nx; half <<= 1) {
-        const double2* tws = tw + (half - 1);  // this stage's table
-        for (int t = threadIdx.x; t < half_n; t += blockDim.x) {
-            const int pos = t & (half - 1);
-            const int i = ((t - pos) << 1) + pos;  // group * 2 half + pos
-            const int j = i + half;
-            double2 w = tws[pos];
-            if (inverse) w.y = -w.y;
-            const double2 u = a[i];
-            const double2 b = a[j];
-            const double2 v = make_double2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
-            a[i] = make_double2(u.x + v.x, u.y + v.y);
-            a[j] = make_double2(u.x - v.x, u.y - v.y);
+// One pass of S consecutive radix-2 stages (S <= 3) done in registers: the
+// 2^S elements b + m d (m < 2^S) are closed under those stages' butterflies,
+// so each thread loads them once, applies the same butterflies with the same
+// twiddles in the same order as stage-by-stage passes (identical results),
+// and stores them once.  d is the smallest half-length of the pass.
+// Decimation in frequency (DIF): halves d 2^(S-1), ..., d, butterfly
+// (u + b, (u - b) w).  Decimation in time: halves d, ..., d 2^(S-1),
+// butterfly (u + b w, u - b w), conjugate twiddles for the inverse.
+// LDS slot of element i of the transform: i with its low 4 bits XORed by a
+// fold of the higher bits (a bijection within each block of 16), so that the
+// strided element sets of the passes and the bit-reversed bins of the
+// spectral step fall in different banks.
+__device__ __forceinline__ int fft_slot(int i) {
+    const int h = i >> 4;
+    return i ^ ((h ^ (h >> 4) ^ (h >> 8)) & 15);
+}
+
+template <int S, bool DIF, bool INV>
+__device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int nx, int d) {
+    constexpr int M = 1 << S;
+    const int groups = nx >> S;
+    for (int t = threadIdx.x; t < groups; t += blockDim.x) {
+        const int p = t & (d - 1);
+        const int b = ((t - p) << S) + p;
+        double2 x[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) x[m] = a[fft_slot(b + m * d)];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const int sh = DIF ? S - 1 - k : k;  // pairs (m, m + 2^sh)
+            const int hk = d << sh;              // this stage's half-length
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if ((m >> sh) & 1) continue;
+                double2 w = tw[hk - 1 + (m & ((1 << sh) - 1)) * d + p];
+                if (INV) w.y = -w.y;
+                const double2 u = x[m];
+                const double2 v = x[m + (1 << sh)];
+                if (DIF) {
+                    const double2 df = make_double2(u.x - v.x, u.y - v.y);
+                    x[m] = make_double2(u.x + v.x, u.y + v.y);
+                    x[m + (1 << sh)] = make_double2(df.x * w.x - df.y * w.y, df.x * w.y + df.y * w.x);
+                } else {
+                    const double2 bw = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+                    x[m] = make_double2(u.x + bw.x, u.y + bw.y);
+                    x[m + (1 << sh)] = make_double2(u.x - bw.x, u.y - bw.y);
+                }
+            }
         }
-        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; ++m) a[fft_slot(b + m * d)] = x[m];
+    }
+    __syncthreads();
+}
+
+// Decimation in time, conjugate twiddles: the inverse transform (without the
+// 1/nx) of a bit-reversed spectrum, natural-order output.
+__device__ __forceinline__ void fft_inverse_dit(double2* a, const double2* tw, int nx, int lg) {
+    int h = 1;
+    for (int left = lg; left > 0;) {
+        const int st = left >= 3 ? 3 : left;
+        if (st == 3)
+            fft_pass<3, false, true>(a, tw, nx, h);
+        else if (st == 2)
+            fft_pass<2, false, true>(a, tw, nx, h);
+        else
+            fft_pass<1, false, true>(a, tw, nx, h);
+        h <<= st;
+        left -= st;
     }
 }
 
 // Decimation in frequency: natural-order input, bit-reversed output (the
 // order the decimation-in-time inverse reads), so neither transform needs a
 // permutation pass (scattered LDS accesses).
-__device__ __forceinline__ void fft_radix2_dif(double2* a, const double2* tw, int nx) {
-    const int half_n = nx >> 1;
-    for (int half = half_n; half >= 1; half >>= 1) {
-        const double2* tws = tw + (half - 1);
-        for (int t = threadIdx.x; t < half_n; t += blockDim.x) {
-            const int pos = t & (half - 1);
-            const int i = ((t - pos) << 1) + pos;
-            const int j = i + half;
-            const double2 w = tws[pos];
-            const double2 u = a[i];
-            const double2 b = a[j];
-            const double2 d = make_double2(u.x - b.x, u.y - b.y);
-            a[i] = make_double2(u.x + b.x, u.y + b.y);
-            a[j] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
-        }
-        __syncthreads();
+__device__ __forceinline__ void fft_forward_dif(double2* a, const double2* tw, int nx, int lg) {
+    int H = nx >> 1;  // largest half-length left
+    for (int left = lg; left > 0;) {
+        const int st = left >= 3 ? 3 : left;
+        const int d = H >> (st - 1);
+        if (st == 3)
+            fft_pass<3, true, false>(a, tw, nx, d);
+        else if (st == 2)
+            fft_pass<2, true, false>(a, tw, nx, d);
+        else
+            fft_pass<1, true, false>(a, tw, nx, d);
+        H = d >> 1;
+        left -= st;
     }
 }
 
-__global__ __launch_bounds__(1024) void poisson1d_fft_kernel(const double* __restrict__ rho, int nx, int lg,
-                                                             const double* __restrict__ inv_kx,
-                                                             const double* __restrict__ inv_kx_sq,
-                                                             double* __restrict__ E, double* __restrict__ phi) {
+constexpr int kFftThreads = 1024;
+constexpr int kFftPer = kFftMaxN / kFftThreads;           // points per thread
+constexpr int kFftBins = kFftMaxN / 2 / kFftThreads + 1;  // bins k <= nx/2 per thread
+
+__global__ __launch_bounds__(kFftThreads) void poisson1d_fft_kernel(const double* __restrict__ rho, int nx, int lg,
+                                                                    const double* __restrict__ inv_kx,
+                                                                    const double* __restrict__ inv_kx_sq,
+                                                                    double* __restrict__ E, double* __restrict__ phi) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double2* a = reinterpret_cast<double2*>(smem_raw);  // [nx]
     double2* tw = a + nx;                                 // [nx - 1]: per-stage tables
     const int half_n = nx >> 1;
-    for (int e = threadIdx.x; e < nx - 1; e += blockDim.x) {
-        const int h = 1 << (31 - __clz(e + 1));  // stage of entry e: h - 1 <= e < 2 h - 1
-        const int pos = e - (h - 1);
-        double sn, cs;
-        sincospi(static_cast<double>(pos) / static_cast<double>(h), &sn, &cs);  // exp(-i pi pos / h)
-        tw[e] = make_double2(cs, -sn);
+    const int t0 = static_cast<int>(threadIdx.x);
+    // this thread's rho values and spectral factors load while the twiddles
+    // are computed
+    double rv[kFftPer], ik[kFftBins], ik2[kFftBins];
+#pragma unroll
+    for (int i = 0; i < kFftPer; ++i) {
+        const int x = t0 + i * kFftThreads;
+        rv[i] = x < nx ? rho[x] : 0.0;
     }
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) a[x] = make_double2(rho[x], 0.0);
+#pragma unroll
+    for (int i = 0; i < kFftBins; ++i) {
+        const int k = t0 + i * kFftThreads;
+        ik[i] = k <= half_n ? inv_kx[k] : 0.0;
+        ik2[i] = k <= half_n ? inv_kx_sq[k] : 0.0;
+    }
+    // the largest stage's table (h = nx/2) from sincospi over its first half
+    // (exp(-i pi (p + h/2) / h) = -i exp(-i pi p / h)); every smaller stage's
+    // entry exp(-i pi pos / h') is the entry pos * (nx/2) / h' of that table
+    const int lh = lg - 1;
+    double2* big = tw + (half_n - 1);
+    for (int p = t0; p < half_n / 2; p += kFftThreads) {
+        double sn, cs;
+        sincospi(ldexp(static_cast<double>(p), -lh), &sn, &cs);  // p / (nx/2), exact
+        big[p] = make_double2(cs, -sn);
+        big[p + half_n / 2] = make_double2(-sn, -cs);
+    }
     __syncthreads();
-    fft_radix2_dif(a, tw, nx);  // X_k at a[brev(k)]
-    auto br = [lg](int k) { return static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - lg)); };
-    // each thread reads bins k <= nx/2 it alone writes; bins above nx/2 are not read
-    for (int k = threadIdx.x; k <= half_n; k += blockDim.x) {
+    for (int e = t0; e < half_n - 1; e += kFftThreads) {
+        const int le = 31 - __clz(e + 1);  // stage of entry e: h' = 2^le, h' - 1 <= e < 2 h' - 1
+        tw[e] = big[(e - ((1 << le) - 1)) << (lh - le)];
+    }
+#pragma unroll
+    for (int i = 0; i < kFftPer; ++i) {
+        const int x = t0 + i * kFftThreads;
+        if (x < nx) a[fft_slot(x)] = make_double2(rv[i], 0.0);
+    }
+    __syncthreads();
+    fft_forward_dif(a, tw, nx, lg);  // X_k at a[brev(k)]
+    auto br = [lg](int k) { return fft_slot(static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - lg))); };
+    // each thread reads bins k <= nx/2 it alone writes; bins above nx/2 are not
+    // read (br gives the LDS slot of bin k)
+#pragma unroll
+    for (int i = 0; i < kFftBins; ++i) {
+        const int k = t0 + i * kFftThreads;
+        if (k > half_n) continue;
         const double2 X = a[br(k)];
-        const double ik = inv_kx[k], ik2 = inv_kx_sq[k];
-        const double er = X.y * ik, ei = -X.x * ik;
-        const double pr = X.x * ik2, pi = X.y * ik2;
+        const double er = X.y * ik[i], ei = -X.x * ik[i];
+        const double pr = X.x * ik2[i], pi = X.y * ik2[i];
         if (k == 0 || k == half_n) {
             a[br(k)] = make_double2(er, pr);
         } else {
@@ -217,11 +302,12 @@ __global__ __launch_bounds__(1024) void poisson1d_fft_kernel(const double* __res
         }
     }
     __syncthreads();
-    fft_radix2(a, tw, nx, true);  // bit-reversed spectrum in, natural order out
+    fft_inverse_dit(a, tw, nx, lg);  // bit-reversed spectrum in, natural order out
     const double inv_n = 1.0 / static_cast<double>(nx);  // exact for a power of two
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-        E[x] = a[x].x * inv_n;
-        phi[x] = a[x].y * inv_n;
+        const double2 z = a[fft_slot(x)];
+        E[x] = z.x * inv_n;
+        phi[x] = z.y * inv_n;
     }
 }
 
@@ -400,7 +486,8 @@ int poisson1d(const double* rho, int64_t nx, const double* inv_kx, const double*
             attr = true;
         }
         const size_t lds = static_cast<size_t>(2 * n - 1) * sizeof(double2);
-        hipLaunchKernelGGL(poisson1d_fft_kernel, dim3(1), dim3(1024), lds, s, rho, n, lg, inv_kx, inv_kx_sq, E, phi);
+        hipLaunchKernelGGL(poisson1d_fft_kernel, dim3(1), dim3(kFftThreads), lds, s, rho, n, lg, inv_kx, inv_kx_sq, E,
+                           phi);
         VP_CHECK_LAUNCH();
         return VP_OK;
     }
