@@ -154,13 +154,15 @@ constexpr int kFftMaxN = 4096;
 // LDS slot of element i of the transform: i with its low 4 bits XORed by a
 // fold of the higher bits (a bijection within each block of 16), so that the
 // strided element sets of the passes and the bit-reversed bins of the
-// spectral step fall in different banks.
+// spectral step fall in different banks (SW; without it, slot i).
+template <bool SW>
 __device__ __forceinline__ int fft_slot(int i) {
+    if (!SW) return i;
     const int h = i >> 4;
     return i ^ ((h ^ (h >> 4) ^ (h >> 8)) & 15);
 }
 
-template <int S, bool DIF, bool INV>
+template <int S, bool DIF, bool INV, bool SW>
 __device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int nx, int d) {
     constexpr int M = 1 << S;
     const int groups = nx >> S;
@@ -169,7 +171,7 @@ __device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int nx, 
         const int b = ((t - p) << S) + p;
         double2 x[M];
 #pragma unroll
-        for (int m = 0; m < M; ++m) x[m] = a[fft_slot(b + m * d)];
+        for (int m = 0; m < M; ++m) x[m] = a[fft_slot<SW>(b + m * d)];
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const int sh = DIF ? S - 1 - k : k;  // pairs (m, m + 2^sh)
@@ -193,23 +195,25 @@ __device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int nx, 
             }
         }
 #pragma unroll
-        for (int m = 0; m < M; ++m) a[fft_slot(b + m * d)] = x[m];
+        for (int m = 0; m < M; ++m) a[fft_slot<SW>(b + m * d)] = x[m];
     }
     __syncthreads();
 }
 
 // Decimation in time, conjugate twiddles: the inverse transform (without the
-// 1/nx) of a bit-reversed spectrum, natural-order output.
+// 1/nx) of a bit-reversed spectrum, natural-order output; at most MAXS
+// stages per pass.
+template <int MAXS>
 __device__ __forceinline__ void fft_inverse_dit(double2* a, const double2* tw, int nx, int lg) {
     int h = 1;
     for (int left = lg; left > 0;) {
-        const int st = left >= 3 ? 3 : left;
+        const int st = left >= MAXS ? MAXS : left;
         if (st == 3)
-            fft_pass<3, false, true>(a, tw, nx, h);
+            fft_pass<3, false, true, (MAXS > 1)>(a, tw, nx, h);
         else if (st == 2)
-            fft_pass<2, false, true>(a, tw, nx, h);
+            fft_pass<2, false, true, (MAXS > 1)>(a, tw, nx, h);
         else
-            fft_pass<1, false, true>(a, tw, nx, h);
+            fft_pass<1, false, true, (MAXS > 1)>(a, tw, nx, h);
         h <<= st;
         left -= st;
     }
@@ -218,17 +222,18 @@ __device__ __forceinline__ void fft_inverse_dit(double2* a, const double2* tw, i
 // Decimation in frequency: natural-order input, bit-reversed output (the
 // order the decimation-in-time inverse reads), so neither transform needs a
 // permutation pass (scattered LDS accesses).
+template <int MAXS>
 __device__ __forceinline__ void fft_forward_dif(double2* a, const double2* tw, int nx, int lg) {
     int H = nx >> 1;  // largest half-length left
     for (int left = lg; left > 0;) {
-        const int st = left >= 3 ? 3 : left;
+        const int st = left >= MAXS ? MAXS : left;
         const int d = H >> (st - 1);
         if (st == 3)
-            fft_pass<3, true, false>(a, tw, nx, d);
+            fft_pass<3, true, false, (MAXS > 1)>(a, tw, nx, d);
         else if (st == 2)
-            fft_pass<2, true, false>(a, tw, nx, d);
+            fft_pass<2, true, false, (MAXS > 1)>(a, tw, nx, d);
         else
-            fft_pass<1, true, false>(a, tw, nx, d);
+            fft_pass<1, true, false, (MAXS > 1)>(a, tw, nx, d);
         H = d >> 1;
         left -= st;
     }
@@ -238,6 +243,9 @@ constexpr int kFftThreads = 1024;
 constexpr int kFftPer = kFftMaxN / kFftThreads;           // points per thread
 constexpr int kFftBins = kFftMaxN / 2 / kFftThreads + 1;  // bins k <= nx/2 per thread
 
+// MAXS: radix-2 stages per LDS pass (3 from 1024 points up; below that the
+// few active threads of a three-stage pass make it slower than single stages).
+template <int MAXS>
 __global__ __launch_bounds__(kFftThreads) void poisson1d_fft_kernel(const double* __restrict__ rho, int nx, int lg,
                                                                     const double* __restrict__ inv_kx,
                                                                     const double* __restrict__ inv_kx_sq,
@@ -261,30 +269,40 @@ __global__ __launch_bounds__(kFftThreads) void poisson1d_fft_kernel(const double
         ik[i] = k <= half_n ? inv_kx[k] : 0.0;
         ik2[i] = k <= half_n ? inv_kx_sq[k] : 0.0;
     }
-    // the largest stage's table (h = nx/2) from sincospi over its first half
-    // (exp(-i pi (p + h/2) / h) = -i exp(-i pi p / h)); every smaller stage's
-    // entry exp(-i pi pos / h') is the entry pos * (nx/2) / h' of that table
     const int lh = lg - 1;
-    double2* big = tw + (half_n - 1);
-    for (int p = t0; p < half_n / 2; p += kFftThreads) {
-        double sn, cs;
-        sincospi(ldexp(static_cast<double>(p), -lh), &sn, &cs);  // p / (nx/2), exact
-        big[p] = make_double2(cs, -sn);
-        big[p + half_n / 2] = make_double2(-sn, -cs);
-    }
-    __syncthreads();
-    for (int e = t0; e < half_n - 1; e += kFftThreads) {
-        const int le = 31 - __clz(e + 1);  // stage of entry e: h' = 2^le, h' - 1 <= e < 2 h' - 1
-        tw[e] = big[(e - ((1 << le) - 1)) << (lh - le)];
+    if (nx - 1 <= kFftThreads) {
+        // one entry per thread: entry e of the stage h' = 2^le is exp(-i pi pos / h')
+        if (t0 < nx - 1) {
+            const int le = 31 - __clz(t0 + 1);  // h' - 1 <= e < 2 h' - 1
+            double sn, cs;
+            sincospi(ldexp(static_cast<double>(t0 - ((1 << le) - 1)), -le), &sn, &cs);
+            tw[t0] = make_double2(cs, -sn);
+        }
+    } else {
+        // the largest stage's table (h = nx/2) from sincospi over its first half
+        // (exp(-i pi (p + h/2) / h) = -i exp(-i pi p / h)); every smaller stage's
+        // entry exp(-i pi pos / h') is the entry pos * (nx/2) / h' of that table
+        double2* big = tw + (half_n - 1);
+        for (int p = t0; p < half_n / 2; p += kFftThreads) {
+            double sn, cs;
+            sincospi(ldexp(static_cast<double>(p), -lh), &sn, &cs);  // p / (nx/2), exact
+            big[p] = make_double2(cs, -sn);
+            big[p + half_n / 2] = make_double2(-sn, -cs);
+        }
+        __syncthreads();
+        for (int e = t0; e < half_n - 1; e += kFftThreads) {
+            const int le = 31 - __clz(e + 1);  // stage of entry e: h' = 2^le, h' - 1 <= e < 2 h' - 1
+            tw[e] = big[(e - ((1 << le) - 1)) << (lh - le)];
+        }
     }
 #pragma unroll
     for (int i = 0; i < kFftPer; ++i) {
         const int x = t0 + i * kFftThreads;
-        if (x < nx) a[fft_slot(x)] = make_double2(rv[i], 0.0);
+        if (x < nx) a[fft_slot<(MAXS > 1)>(x)] = make_double2(rv[i], 0.0);
     }
     __syncthreads();
-    fft_forward_dif(a, tw, nx, lg);  // X_k at a[brev(k)]
-    auto br = [lg](int k) { return fft_slot(static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - lg))); };
+    fft_forward_dif<MAXS>(a, tw, nx, lg);  // X_k at a[brev(k)]
+    auto br = [lg](int k) { return fft_slot<(MAXS > 1)>(static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - lg))); };
     // each thread reads bins k <= nx/2 it alone writes; bins above nx/2 are not
     // read (br gives the LDS slot of bin k)
 #pragma unroll
@@ -302,10 +320,10 @@ __global__ __launch_bounds__(kFftThreads) void poisson1d_fft_kernel(const double
         }
     }
     __syncthreads();
-    fft_inverse_dit(a, tw, nx, lg);  // bit-reversed spectrum in, natural order out
+    fft_inverse_dit<MAXS>(a, tw, nx, lg);  // bit-reversed spectrum in, natural order out
     const double inv_n = 1.0 / static_cast<double>(nx);  // exact for a power of two
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-        const double2 z = a[fft_slot(x)];
+        const double2 z = a[fft_slot<(MAXS > 1)>(x)];
         E[x] = z.x * inv_n;
         phi[x] = z.y * inv_n;
     }
@@ -480,14 +498,21 @@ int poisson1d(const double* rho, int64_t nx, const double* inv_kx, const double*
         while ((1 << lg) < n) ++lg;
         static bool attr = false;
         if (!attr) {
-            VP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&poisson1d_fft_kernel),
+            VP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&poisson1d_fft_kernel<1>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           static_cast<int>(2 * kFftMaxN * sizeof(double2))));
+            VP_TRY_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&poisson1d_fft_kernel<3>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            static_cast<int>(2 * kFftMaxN * sizeof(double2))));
             attr = true;
         }
         const size_t lds = static_cast<size_t>(2 * n - 1) * sizeof(double2);
-        hipLaunchKernelGGL(poisson1d_fft_kernel, dim3(1), dim3(kFftThreads), lds, s, rho, n, lg, inv_kx, inv_kx_sq, E,
-                           phi);
+        if (n >= 1024)
+            hipLaunchKernelGGL(poisson1d_fft_kernel<3>, dim3(1), dim3(kFftThreads), lds, s, rho, n, lg, inv_kx,
+                               inv_kx_sq, E, phi);
+        else
+            hipLaunchKernelGGL(poisson1d_fft_kernel<1>, dim3(1), dim3(kFftThreads), lds, s, rho, n, lg, inv_kx,
+                               inv_kx_sq, E, phi);
         VP_CHECK_LAUNCH();
         return VP_OK;
     }
