@@ -250,6 +250,8 @@ struct LineSweep {
     }
 };
 
+constexpr int kTilePrefetch = 8;  // whole-line tiles: elements per thread loaded before the table fill
+
 // Tile = n rows (the swept axis) x C columns (contiguous inner elements).
 // TAB (sweep parameters constant along each line): for the departure form,
 // binade weight tables of the C lines after the tile in LDS; for the shift
@@ -270,14 +272,27 @@ __global__ __launch_bounds__(256) void advect_strided_kernel(
     const int64_t base = o * static_cast<int64_t>(n) * inner + i0 + c;
     const int64_t ooff = split_offset(o, vm.od, vm.os, vm.no);
     T* tab = tile + static_cast<int64_t>(n) * C;
+    // this thread's first kTilePrefetch rows load while the tables are computed
+    T pre[kTilePrefetch];
+#pragma unroll
+    for (int k = 0; k < kTilePrefetch; ++k) {
+        const int r = r0 + k * rstep;
+        pre[k] = (cval && r < n) ? fin[base + r * inner] : T(0);
+    }
     if (TAB && FORM == kDeparture) {
         const int64_t lim = inner - i0;
         fill_binade_tables<T>(tab, C, kTabStride, [&](int line) {
             return line < lim ? P.vel[ooff + split_offset(i0 + line, vm.id, vm.is, vm.ni)] : 0.0;
         }, P.dt, P.dx);
     }
-    if (cval)
-        for (int r = r0; r < n; r += rstep) tile[r * C + c] = fin[base + r * inner];
+    if (cval) {
+#pragma unroll
+        for (int k = 0; k < kTilePrefetch; ++k) {
+            const int r = r0 + k * rstep;
+            if (r < n) tile[r * C + c] = pre[k];
+        }
+        for (int r = r0 + kTilePrefetch * rstep; r < n; r += rstep) tile[r * C + c] = fin[base + r * inner];
+    }
     __syncthreads();
     if (!cval) return;
     const int64_t voff = ooff + split_offset(i0 + c, vm.id, vm.is, vm.ni);
@@ -548,11 +563,24 @@ __global__ __launch_bounds__(256) void advect_contig_kernel(
     const int64_t cnt = nl * n;
     const T* src = fin + l0 * n;
     T* tab = tile + static_cast<int64_t>(L) * n;
+    // this thread's first kTilePrefetch elements load while the tables are computed
+    T pre[kTilePrefetch];
+#pragma unroll
+    for (int k = 0; k < kTilePrefetch; ++k) {
+        const int64_t idx = threadIdx.x + static_cast<int64_t>(k) * blockDim.x;
+        pre[k] = idx < cnt ? src[idx] : T(0);
+    }
     if (TAB && FORM == kDeparture)
         fill_binade_tables<T>(tab, static_cast<int>(nl), kBinades * 6, [&](int line) {
             return P.vel[split_offset(l0 + line, vm.od, vm.os, vm.no)];
         }, P.dt, P.dx);
-    for (int64_t idx = threadIdx.x; idx < cnt; idx += blockDim.x) tile[idx] = src[idx];
+#pragma unroll
+    for (int k = 0; k < kTilePrefetch; ++k) {
+        const int64_t idx = threadIdx.x + static_cast<int64_t>(k) * blockDim.x;
+        if (idx < cnt) tile[idx] = pre[k];
+    }
+    for (int64_t idx = threadIdx.x + static_cast<int64_t>(kTilePrefetch) * blockDim.x; idx < cnt; idx += blockDim.x)
+        tile[idx] = src[idx];
     __syncthreads();
     const int lane = threadIdx.x % kWave;
     const int wid = threadIdx.x / kWave;

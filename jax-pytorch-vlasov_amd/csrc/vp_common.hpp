@@ -80,7 +80,12 @@ __device__ __forceinline__ Departure departure(int n, double vel, double dt, dou
     return d;
 }
 
+// j mod n in [0, n).  Taps lie within one period of the line almost always:
+// a compare and an add there, the 64-bit remainder (tens of instructions)
+// only beyond.
 __device__ __forceinline__ int wrap_index(int64_t j, int n) {
+    const int64_t nn = n;
+    if (j >= -nn && j < 2 * nn) return static_cast<int>(j < 0 ? j + nn : (j >= nn ? j - nn : j));
     int64_t r = j % n;
     return static_cast<int>(r < 0 ? r + n : r);
 }
