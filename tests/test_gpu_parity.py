@@ -143,9 +143,12 @@ def test_advect_dirichlet_shift_out_is_zero(device):
     assert torch.count_nonzero(out).item() == 0
 
 
-@pytest.mark.parametrize("nx", [128, 512, 1000, 4096, 4098])
+@pytest.mark.parametrize("nx", [8, 16, 128, 256, 512, 1000, 1024, 2048, 4096, 4098])
 def test_rho_and_poisson_1d(device, nx):
-    """One-workgroup solve (nx < 512, > 4096) and many-workgroup solve."""
+    """Power-of-two nx up to 4096: FFTs, one stage per LDS pass below 1024
+    and three from 1024 (ending on a pass of one stage at 1024, two at 2048,
+    none at 4096); other nx: the many-workgroup DFT (512..4096) or the
+    one-workgroup DFT."""
     v1 = _v1()
     g = vo.grid_1d(nx, 96, 4 * math.pi, 6.0)
     rng = np.random.default_rng(4)
