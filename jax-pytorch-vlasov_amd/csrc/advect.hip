@@ -417,15 +417,13 @@ __global__ __launch_bounds__(256) void advect_seg_kernel(const T* __restrict__ f
         pre[k] = rr < cnt ? staged(rr) : T(0);
     }
     if (FORM == kDeparture) {
-        for (int e = threadIdx.x; e < KC * kSegBinades; e += blockDim.x) {
-            const int line = e / kSegBinades, q = e % kSegBinades;
+        // code-major over the threads: the KC * nq entries in use occupy the
+        // first threads, so whole waves skip the divisions (slots >= nq are
+        // never read)
+        for (int e = threadIdx.x; e < KC * nq; e += blockDim.x) {
+            const int q = e / KC, line = e % KC;
             T w[6];
-            if (q < nq) {
-                code_weights<T>(keys[line], q0 + q, w);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) w[k] = T(0);
-            }
+            code_weights<T>(keys[line], q0 + q, w);
 #pragma unroll
             for (int k = 0; k < 6; ++k) tab[line * TS + q * 6 + k] = w[k];
         }
