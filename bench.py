@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true", help="skip training the Fourier-operator surrogate")
     ap.add_argument("--no-reference-configs", action="store_true",
                     help="skip the per-step times on the configurations the reference published")
-    ap.add_argument("--surrogate-iters", type=int, default=24000)
+    ap.add_argument("--surrogate-iters", type=int, default=36000)
     ap.add_argument("--stream-only", action="store_true",
                     help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
     return ap.parse_args()
