@@ -62,6 +62,20 @@ int vp_advect_f64(const double* f_in, double* f_out, const int64_t shape[4], int
                   const double* vel, const int64_t vel_strides[4],
                   double dt, double dx, int periodic, vp_stream_t stream);
 
+/* Kokkos-semantics sweep along `axis` (advect_1d_periodic / _dirichlet,
+ * simulations/vlasov2d_2v/kokkos/src/vlasov_poisson.hpp:293-339, with
+ * lagrange_interp_5_uniform :244-291), for cross-checking against Kokkos
+ * runs: departure point grid[i] - vel*dt in physical coordinates (grid has
+ * shape[axis] points; vel indexed as in vp_advect_f64); periodic: wrapped
+ * by fmod(dp - xmin, L), index floor(. * inv_dx); Dirichlet: 0 outside
+ * [xmin, xmin + L], taps outside [0, n) read as 0; Horner-form weights.
+ * Not the parity target of the rest of the library (which follows the
+ * PyTorch solvers); f_in and f_out must not overlap. */
+int vp_advect_kokkos_f64(const double* f_in, double* f_out, const int64_t shape[4], int axis,
+                         const double* grid, const double* vel, const int64_t vel_strides[4],
+                         double dt, double L, double xmin, double inv_dx, int periodic,
+                         vp_stream_t stream);
+
 /* FP32-storage form: f in float32, departure/weights in float64, the
  * 6-tap sum in float32.  No reference counterpart (the reference's float32
  * mode computes everything in float32). */

@@ -29,6 +29,8 @@ _SIGS = {
     "vp_advect_f64": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, I64x4, c_dbl, c_dbl, c_int, c_vp]),
     "vp_advect_f32": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, I64x4, c_dbl, c_dbl, c_int, c_vp]),
     "vp_advect_reflected_f64": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, I64x4, c_dbl, c_dbl, c_int, c_int, c_vp]),
+    "vp_advect_kokkos_f64": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, c_vp, I64x4, c_dbl, c_dbl, c_dbl, c_dbl, c_int,
+                                     c_vp]),
     "vp_advect_static_f64": (c_int, [c_vp, c_vp, I64x4, c_int, c_vp, c_vp, c_i64, I64x4, c_vp]),
     "vp_row_sum_f64": (c_int, [c_vp, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_vp, c_vp]),
     "vp_row_sum_f32": (c_int, [c_vp, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_vp, c_vp]),
