@@ -20,10 +20,14 @@ How each function is pinned (tests/test_oracle_*.py):
   alpha in {0.1, 0.25, 0.5, 0.75, 0.9}, alpha=0 returns the centre tap,
   partition of unity (P1 tests/test_interpolation.py:4-65), and equality with
   the generic Lagrange product form (P2:210-235).
-* ``advect_rows`` / ``advect_axis``: brute-force dense interpolation matrix on
-  tiny grids, integer shifts are exact permutations, periodic drift conserves
-  the sum, analytic Gaussian shifts (P1 tests/test_advection.py:50-195), fully
-  out-of-domain Dirichlet shift gives zero.
+* ``advect_rows`` / ``advect_axis_direct``: brute-force dense interpolation
+  matrix on tiny grids, integer shifts are exact permutations, periodic drift
+  conserves the sum, analytic Gaussian shifts (P1 tests/test_advection.py:50-195),
+  fully out-of-domain Dirichlet shift gives zero.
+* ``advect_axis`` (the line form the steps use: weights per (velocity, index),
+  one gather per tap, large arrays in threaded blocks): array-equal to
+  ``advect_axis_direct`` for every velocity layout of the steps, multi-cell
+  and out-of-domain shifts, blocks forced on (tests/test_oracle_interp_advect.py).
 * ``solve_poisson_real_1d`` / ``compute_poisson_real``: single-mode closed
   forms (P1 tests/test_poisson.py:6-64, P2 tests/test_poisson.py:6-75), a dense
   DFT matrix restatement of rfft/irfft including the dropped imaginary parts
@@ -137,11 +141,103 @@ def advect_rows(f_batch, velocities, dt, dx, periodic=True):
 
 
 def advect_axis(f, velocities, axis, dt, dx, periodic=True):
-    """``advect_1d`` (P2:356-413): sweep of a 4D array along ``axis``.
+    """``advect_1d`` (P2:356-413): sweep of an array along ``axis``.
 
     ``velocities`` broadcasts to the batch shape (all dims but ``axis``, in
-    order), as in the reference.
+    order), as in the reference, so the velocity is constant along every
+    swept line.  Same arithmetic as ``advect_axis_direct`` element for
+    element (P2:330-353): departure ``i - (v*dt)/dx``, ``floor``, fraction,
+    power-form weights, taps summed left to right, remainder or clamp+mask;
+    the weights and tap indices are evaluated once on the compact
+    (velocity, index) shape instead of per element, and the taps are
+    gathered one at a time, which keeps the memory at a few copies of f
+    (the direct form builds [B, N, 6] int64 index arrays).  Results are
+    bit-identical to ``advect_axis_direct`` (tests/test_oracle_interp_advect.py).
     """
+    f = np.asarray(f, dtype=np.float64)
+    n = f.shape[axis]
+    nb = f.ndim - 1
+    vel = np.asarray(velocities, dtype=np.float64)
+    if vel.ndim > nb:
+        raise ValueError("velocities have more dimensions than the batch")
+    vel = vel.reshape((1,) * (nb - vel.ndim) + vel.shape)
+    np.broadcast_shapes(vel.shape, f.shape[:axis] + f.shape[axis + 1:])  # raises like the reference's broadcast
+    # compact shape with the swept axis in place: the gathers below then read
+    # whole contiguous rows of the trailing axes
+    displacement = np.expand_dims((vel * dt) / dx, axis)
+    ishape = [1] * f.ndim
+    ishape[axis] = n
+    index = np.arange(n, dtype=np.float64).reshape(ishape)
+    chunks = _line_chunks(f, axis)
+    if len(chunks) == 1:
+        return np.ascontiguousarray(_advect_block(f, displacement, index, axis, n, periodic))
+    # large arrays: independent blocks along a non-swept axis on a thread pool
+    # (numpy releases the GIL in gathers and ufuncs); every element is computed
+    # by the same operations as in one block
+    out = np.empty_like(f)
+
+    def run(sl):
+        d = displacement[sl] if all(displacement.shape[i] > 1 for i in range(f.ndim) if sl[i] != slice(None)) \
+            else displacement
+        out[sl] = _advect_block(f[sl], d, index, axis, n, periodic)
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=_oracle_threads()) as ex:
+        list(ex.map(run, chunks))
+    return out
+
+
+def _oracle_threads():
+    import os
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+# arrays from this many elements up are swept in blocks on a thread pool
+CHUNK_MIN_ELEMS = 1 << 21
+
+
+def _line_chunks(f, axis):
+    """Slices of f along its largest non-swept axis (one slice for small f)."""
+    if f.size < CHUNK_MIN_ELEMS or f.ndim < 2:
+        return [tuple(slice(None) for _ in range(f.ndim))]
+    cand = [i for i in range(f.ndim) if i != axis]
+    c = max(cand, key=lambda i: f.shape[i])
+    parts = min(f.shape[c], 4 * _oracle_threads())
+    bounds = [f.shape[c] * p // parts for p in range(parts + 1)]
+    out = []
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        if b > a:
+            sl = [slice(None)] * f.ndim
+            sl[c] = slice(a, b)
+            out.append(tuple(sl))
+    return out
+
+
+def _advect_block(f, displacement, index, axis, n, periodic):
+    depart = index - displacement
+    idx_left = np.floor(depart)
+    w = lagrange5_weights(depart - idx_left)
+    idx_left = idx_left.astype(np.int64)
+    out = None
+    for k in range(6):
+        j = idx_left + STENCIL_OFFSETS[k]
+        if periodic:
+            vals = np.take_along_axis(f, np.broadcast_to(np.remainder(j, n), f.shape), axis=axis)
+        else:
+            valid = ((j >= 0) & (j < n)).astype(np.float64)
+            vals = np.take_along_axis(f, np.broadcast_to(np.clip(j, 0, n - 1), f.shape), axis=axis) * valid
+        term = w[k] * vals
+        out = term if out is None else out + term
+    return out
+
+
+def advect_axis_direct(f, velocities, axis, dt, dx, periodic=True):
+    """``advect_1d`` (P2:356-413) as the reference writes it: permute the
+    swept axis last, reshape to [B, N], ``advect_rows`` with one velocity per
+    row, permute back.  ``velocities`` broadcasts to the batch shape."""
     f = np.asarray(f, dtype=np.float64)
     perm = list(range(f.ndim))
     perm.append(perm.pop(axis))

@@ -147,3 +147,50 @@ def test_advect_axis_equals_rows_on_moved_axis(axis):
         prof = np.sin(2 * np.pi * np.arange(5) / 5)[:, None, None, None] * np.ones(shape)
         o2 = vo.advect_axis(prof, np.zeros(batch), 0, 0.1, 0.05, periodic=True)
         np.testing.assert_array_equal(o2, prof)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_line_form_equals_direct_form_on_broadcast_velocities(periodic):
+    """advect_axis (weights on the compact velocity shape) == advect_axis_direct
+    (per-element weights on [B, N, 6] taps) for the velocity layouts of the
+    steps: drift (vx[None, :, None] over axis 0, vy[None, None, :] over axis
+    1), kick (E[:, :, None] over axes 2 and 3), 1D rows; displacements up to
+    many cells and beyond the line (Dirichlet out of domain), both signs."""
+    rng = np.random.default_rng(7)
+    shape = (6, 5, 9, 8)
+    f = rng.random(shape)
+    dx = 0.05
+    big = np.concatenate([rng.normal(size=20) * 0.3, [0.0, 1e-18, -1e-18, 2.5, -2.5, 11.0, -11.0, 40.0]]) * dx / 0.1
+    cases = [
+        (0, rng.choice(big, size=9)[None, :, None]),
+        (1, rng.choice(big, size=8)[None, None, :]),
+        (2, rng.choice(big, size=(6, 5))[:, :, None]),
+        (3, rng.choice(big, size=(6, 5))[:, :, None]),
+        (0, rng.choice(big, size=(5, 9, 8))),
+    ]
+    for axis, vel in cases:
+        a = vo.advect_axis(f, vel, axis, 0.1, dx, periodic=periodic)
+        b = vo.advect_axis_direct(f, vel, axis, 0.1, dx, periodic=periodic)
+        np.testing.assert_array_equal(a, b)
+    f2 = rng.random((7, 11))
+    vel = rng.choice(big, size=7)
+    np.testing.assert_array_equal(vo.advect_axis(f2, vel, 1, 0.1, dx, periodic),
+                                  vo.advect_rows(f2, vel, 0.1, dx, periodic))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_threaded_blocks_equal_direct_form(monkeypatch, periodic):
+    """The blocked, threaded sweep of large arrays computes every element by
+    the same operations: forced on for a small array, it equals the direct
+    form for every velocity layout (blocks along the velocity's own axis and
+    along a broadcast one)."""
+    monkeypatch.setattr(vo, "CHUNK_MIN_ELEMS", 1)
+    rng = np.random.default_rng(8)
+    shape = (7, 9, 6, 10)
+    f = rng.random(shape)
+    dx = 0.05
+    for axis, vel in [(0, rng.normal(size=6)[None, :, None] * 3), (1, rng.normal(size=10)[None, None, :] * 3),
+                      (2, rng.normal(size=(7, 9))[:, :, None] * 3), (3, rng.normal(size=(7, 9))[:, :, None] * 3),
+                      (1, rng.normal(size=(7, 6, 10)) * 3)]:
+        np.testing.assert_array_equal(vo.advect_axis(f, vel, axis, 0.1, dx, periodic),
+                                      vo.advect_axis_direct(f, vel, axis, 0.1, dx, periodic))
