@@ -34,7 +34,8 @@ enum {
     VP_ERR_ARG = -1,        /* null pointer, bad axis / flag                 */
     VP_ERR_SHAPE = -2,      /* size out of the supported range              */
     VP_ERR_WORKSPACE = -3,  /* ws too small (see the *_workspace query)     */
-    VP_ERR_UNSUPPORTED = -4 /* shape/variant this library has no kernel for */
+    VP_ERR_UNSUPPORTED = -4, /* shape/variant this library has no kernel for */
+    VP_ERR_SCRATCH = -5      /* problem larger than VP_STATIC_SCRATCH_BYTES  */
 };
 
 const char* vp_error_string(int code);
@@ -145,6 +146,26 @@ int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, 
                      const double* v, const double* inv_kx, const double* inv_kx_sq,
                      double dx, double dv, int physics_mode, double* E, double* rho,
                      double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
+
+/* vp_step_1d1v_f64 without the workspace argument and without allocating:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The scratch is a static device array of
+ * VP_STATIC_SCRATCH_BYTES in the library, one per GPU, held until the
+ * process ends; problems needing more (vp_step_1d1v_workspace(nx, nv) >
+ * VP_STATIC_SCRATCH_BYTES: up to nx * nv ~ 1.1 M cells) return
+ * VP_ERR_SCRATCH and launch nothing.  The scratch holds only intermediates
+ * that each call writes before reading them.  Calls on one GPU share its
+ * array: calls that may overlap (on different streams of the same GPU, or
+ * from several threads without ordering) race; calls ordered on one stream
+ * do not.  The device is the one current at the call; the first call on a
+ * device looks the array's address up (make it before a graph capture
+ * starts).  Results are those of vp_step_1d1v_f64. */
+#define VP_STATIC_SCRATCH_BYTES ((size_t)9 << 20)
+int vp_step_1d1v_f64_noalloc(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
+                             const double* v, const double* inv_kx, const double* inv_kx_sq,
+                             double dx, double dv, int physics_mode, double* E, double* rho,
+                             double* phi, vp_stream_t stream);
 
 /* vlasov_step_matrix_diag (P1:625-665, solver 1) and
  * vlasov_step_gather_free_impl (P1:668-701, solver 2) in one call:

@@ -1,5 +1,7 @@
 // Full Strang steps composed from the sweep and field kernels, plus the
 // small C-ABI utilities (error strings, version).
+#include <mutex>
+
 #include "fast2d.hpp"
 
 namespace vp {
@@ -33,6 +35,9 @@ extern "C" const char* vp_error_string(int code) {
         case VP_ERR_SHAPE: return "array size outside the supported range";
         case VP_ERR_WORKSPACE: return "workspace too small";
         case VP_ERR_UNSUPPORTED: return "shape or variant not supported by this library";
+        case VP_ERR_SCRATCH:
+            return "problem needs more scratch than the allocation-free entry point holds "
+                   "(VP_STATIC_SCRATCH_BYTES = 9 MiB: vp_step_1d1v_workspace(nx, nv) must fit)";
         default: break;
     }
     if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));
@@ -79,6 +84,53 @@ extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, i
     VP_TRY(vp_advect_f64(f_out, tmp, shape, 3, E, vs_v, dt, dv, 0, stream));
     VP_TRY(vp_advect_f64(tmp, f_out, shape, 2, v, vs_x, half, dx, 1, stream));
     return VP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free 1D1V step: the workspace of vp_step_1d1v_f64 lives in a
+// static device array (one per GPU, for the life of the process).  In host
+// code `scratch` names a host-side stand-in, never the device memory: its
+// device address is looked up once per device with hipGetSymbolAddress and
+// only that address is passed on.
+// ---------------------------------------------------------------------------
+namespace {
+__device__ __align__(256) unsigned char scratch[VP_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device, with it current).
+int static_scratch(void** out) {
+    int dev = -1;
+    VP_TRY_HIP(hipGetDevice(&dev));
+    VP_RETURN_IF(dev < 0 || dev >= kMaxDevices, VP_ERR_ARG);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev] == nullptr) {
+        void* p = nullptr;
+        VP_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        VP_RETURN_IF(p == nullptr, VP_ERR_ARG);
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return VP_OK;
+}
+}  // namespace
+
+extern "C" int vp_step_1d1v_f64_noalloc(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
+                                        const double* v, const double* inv_kx, const double* inv_kx_sq, double dx,
+                                        double dv, int physics_mode, double* E, double* rho, double* phi,
+                                        vp_stream_t stream) {
+    VP_RETURN_IF(nx < 1 || nv < 1 || nx > (1 << 24) || nv > (1 << 24), VP_ERR_SHAPE);
+    VP_RETURN_IF(vp_step_1d1v_workspace(nx, nv) > VP_STATIC_SCRATCH_BYTES, VP_ERR_SCRATCH);
+    void* ws = nullptr;
+    VP_TRY(static_scratch(&ws));
+    // every scratch region (the f copy between the kick and the closing
+    // drift, the Poisson intermediates) is written before it is read within
+    // the call, so nothing has to be cleared between calls
+    return vp_step_1d1v_f64(f_in, f_out, nx, nv, dt, v, inv_kx, inv_kx_sq, dx, dv, physics_mode, E, rho, phi, ws,
+                            VP_STATIC_SCRATCH_BYTES, stream);
 }
 
 // Solver-1 and solver-2 steps in one call: the X sweeps are the reflected

@@ -61,3 +61,14 @@ def test_argument_validation_without_gpu():
     assert L.vp_advect_f64(8, 16, _lib.i64x4((1, 1, 1, 9000)), 3, 24, vs, 0.1, 0.1, 1, None) == -4
     assert L.vp_poisson_2d_f64(8, 16, 16, 8, 8, 8, None, None, None, None, 8, 1, None) == -3
     assert L.vp_poisson_2d_f64(8, 1024, 16, 8, 8, 8, None, None, None, None, 8, 1 << 30, None) == -2
+
+
+def test_noalloc_step_rejects_oversized_problems_without_gpu():
+    """The allocation-free step checks the size against its static scratch
+    before any HIP call."""
+    from vlasov_amd import _lib
+    L = _lib.lib()
+    rc = L.vp_step_1d1v_f64_noalloc(8, 16, 1024, 2048, 0.1, 8, 8, 8, 0.1, 0.1, 1, 8, 8, 8, None)
+    assert rc == -5
+    assert b"scratch" in L.vp_error_string(rc)
+    assert L.vp_step_1d1v_f64_noalloc(8, 16, 0, 128, 0.1, 8, 8, 8, 0.1, 0.1, 1, 8, 8, 8, None) == -2
