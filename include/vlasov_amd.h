@@ -432,7 +432,10 @@ enum {
     VP_KNOB_F32_PAIRS = 11,    /* float32 X drift: pairs of vy columns moved
                                   as 8-byte elements, 1 = never, 2 = always
                                   (0 = for the two-pass drift only)            */
-    VP_KNOB_COUNT = 12
+    VP_KNOB_POISSON_DFT = 12,  /* 1: 2D Poisson by the exact-twiddle DFT kernels
+                                  also for power-of-two grids (default: radix-2
+                                  FFTs there); results agree to rounding      */
+    VP_KNOB_COUNT = 13
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

@@ -987,7 +987,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
 // Host side
 // ---------------------------------------------------------------------------
 // Launch-shape knobs (vp_tune_set); 0 = automatic choice.
-static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
 

@@ -460,6 +460,238 @@ __global__ __launch_bounds__(256) void p2d_rows_inv(const double2* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// 2D, power-of-two nx and ny (8..512): the same transforms as radix-2 FFTs.
+// Three kernels as above (rows forward, columns, rows inverse); each
+// workgroup holds several lines in LDS and runs up to three butterfly stages
+// per pass through LDS on all of them at once.  Forward transforms are
+// decimation in frequency (natural order in, bit-reversed out), inverses
+// decimation in time with conjugate twiddles (bit-reversed in, natural
+// out), so no permutation pass is needed; twiddles come from one table per
+// stage, tw[h - 1 + p] = exp(-i pi p / h), h the stage's half-length
+// (sincospi of exact arguments).  Agrees with the DFT kernels to rounding.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void fill_stage_twiddles(double2* tw, int n) {
+    for (int e = threadIdx.x; e < n - 1; e += blockDim.x) {
+        const int le = 31 - __clz(e + 1);  // stage h = 2^le holds entries h - 1 .. 2h - 2
+        double sn, cs;
+        sincospi(ldexp(static_cast<double>(e - ((1 << le) - 1)), -le), &sn, &cs);
+        tw[e] = make_double2(cs, -sn);
+    }
+}
+
+// S consecutive stages on `lines` lines of n points stored back to back at
+// a (see fft_pass: the 2^S elements b + m d of a line are closed under
+// those stages), then a barrier.
+template <int S, bool DIF, bool INV>
+__device__ __forceinline__ void lines_pass(double2* a, int lines, int n, int d, const double2* tw) {
+    constexpr int M = 1 << S;
+    const int gshift = 31 - __clz(n >> S);  // groups per line = n >> S, a power of two
+    const int total = lines << gshift;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int line = idx >> gshift;
+        const int t = idx - (line << gshift);
+        const int p = t & (d - 1);
+        const int b = ((t - p) << S) + p;
+        double2* base = a + line * n;
+        double2 x[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) x[m] = base[b + m * d];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const int sh = DIF ? S - 1 - k : k;
+            const int hk = d << sh;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                if ((m >> sh) & 1) continue;
+                double2 w = tw[hk - 1 + (m & ((1 << sh) - 1)) * d + p];
+                if (INV) w.y = -w.y;
+                const double2 u = x[m];
+                const double2 v = x[m + (1 << sh)];
+                if (DIF) {
+                    const double2 df = make_double2(u.x - v.x, u.y - v.y);
+                    x[m] = make_double2(u.x + v.x, u.y + v.y);
+                    x[m + (1 << sh)] = make_double2(df.x * w.x - df.y * w.y, df.x * w.y + df.y * w.x);
+                } else {
+                    const double2 bw = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+                    x[m] = make_double2(u.x + bw.x, u.y + bw.y);
+                    x[m + (1 << sh)] = make_double2(u.x - bw.x, u.y - bw.y);
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) base[b + m * d] = x[m];
+    }
+    __syncthreads();
+}
+
+// Forward transform of every line (natural order in, X_k at brev(k) out).
+__device__ __forceinline__ void lines_forward(double2* a, int lines, int n, int lg, const double2* tw) {
+    int H = n >> 1;
+    for (int left = lg; left > 0;) {
+        const int st = left >= 3 ? 3 : left;
+        const int d = H >> (st - 1);
+        if (st == 3)
+            lines_pass<3, true, false>(a, lines, n, d, tw);
+        else if (st == 2)
+            lines_pass<2, true, false>(a, lines, n, d, tw);
+        else
+            lines_pass<1, true, false>(a, lines, n, d, tw);
+        H = d >> 1;
+        left -= st;
+    }
+}
+
+// Inverse transform without the 1/n (bit-reversed in, natural order out).
+__device__ __forceinline__ void lines_inverse(double2* a, int lines, int n, int lg, const double2* tw) {
+    int h = 1;
+    for (int left = lg; left > 0;) {
+        const int st = left >= 3 ? 3 : left;
+        if (st == 3)
+            lines_pass<3, false, true>(a, lines, n, h, tw);
+        else if (st == 2)
+            lines_pass<2, false, true>(a, lines, n, h, tw);
+        else
+            lines_pass<1, false, true>(a, lines, n, h, tw);
+        h <<= st;
+        left -= st;
+    }
+}
+
+__device__ __forceinline__ int brev_lg(int k, int lg) {
+    return static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - lg));
+}
+
+// rows x0 .. x0 + rows - 1: rho (from `rho` or merged partial sums), delta_rho,
+// rfft along y -> R[x][k], k <= ny/2.  The partial sums of a point are
+// loaded eight at a time and merged in part order (the order of p2d_rows_fwd).
+__global__ __launch_bounds__(256) void p2d_fft_rows_fwd(const double* __restrict__ rho,
+                                                        const double2* __restrict__ partial, int nparts, double dvx,
+                                                        double dvy, int nx, int ny, int lgy, int rows,
+                                                        const double* __restrict__ rho_shift,
+                                                        double* __restrict__ delta_rho, double2* __restrict__ R) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* tw = reinterpret_cast<double2*>(smem_raw);  // [ny - 1]
+    double2* a = tw + ny;                                  // [rows][ny]
+    const int x0 = blockIdx.x * rows;
+    fill_stage_twiddles(tw, ny);
+    const int64_t npoints = static_cast<int64_t>(nx) * ny;
+    for (int idx = threadIdx.x; idx < rows * ny; idx += blockDim.x) {
+        const int64_t pt = static_cast<int64_t>(x0) * ny + idx;  // x * ny + y
+        double rv;
+        if (partial) {
+            const double2* pp = partial + pt;  // layout [part][x * ny + y]
+            KSum k{0.0, 0.0};
+            int i = 0;
+            for (; i + 8 <= nparts; i += 8) {
+                double2 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = pp[(i + u) * npoints];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) ksum_merge(k, KSum{v[u].x, v[u].y});
+            }
+            for (; i < nparts; ++i) ksum_merge(k, KSum{pp[i * npoints].x, pp[i * npoints].y});
+            rv = ((k.s + k.c) * dvx) * dvy;
+        } else {
+            rv = rho[pt];
+        }
+        const double d = rho_shift ? rv + rho_shift[0] : rv - 1.0;
+        a[idx] = make_double2(d, 0.0);
+        if (delta_rho) delta_rho[pt] = d;
+    }
+    __syncthreads();
+    lines_forward(a, rows, ny, lgy, tw);
+    const int nky = ny / 2 + 1;
+    for (int idx = threadIdx.x; idx < rows * nky; idx += blockDim.x) {
+        const int r = idx / nky;
+        const int k = idx - r * nky;
+        R[static_cast<int64_t>(x0 + r) * nky + k] = a[r * ny + brev_lg(k, lgy)];
+    }
+}
+
+// column ky_i: c2c FFT along x, phi_k = R_k inv_ksq, Ex_k = -i kx phi_k,
+// Ey_k = -i ky phi_k, three inverse FFTs along x (scaled 1/nx) -> T[q][x][ky_i].
+__global__ __launch_bounds__(256) void p2d_fft_cols(const double2* __restrict__ R, int nx, int ny, int lgx,
+                                                    const double* __restrict__ kx, const double* __restrict__ ky,
+                                                    const double* __restrict__ inv_ksq, double2* __restrict__ T) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* tw = reinterpret_cast<double2*>(smem_raw);  // [nx - 1]
+    double2* a = tw + nx;                                  // [4][nx]: the column, then phi_k, Ex_k, Ey_k
+    const int nky = ny / 2 + 1;
+    const int ky_i = blockIdx.x;
+    fill_stage_twiddles(tw, nx);
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) a[x] = R[static_cast<int64_t>(x) * nky + ky_i];
+    __syncthreads();
+    lines_forward(a, 1, nx, lgx, tw);
+    const double kyv = ky[ky_i];
+    for (int s = threadIdx.x; s < nx; s += blockDim.x) {
+        const int k = brev_lg(s, lgx);  // slot s holds bin k
+        const double2 X = a[s];
+        const double iks = inv_ksq[static_cast<int64_t>(k) * nky + ky_i];
+        const double pr = X.x * iks, pi = X.y * iks;
+        const double kxv = kx[k];
+        a[nx + s] = make_double2(pr, pi);                  // phi_k
+        a[2 * nx + s] = make_double2(pi * kxv, -pr * kxv);  // Ex_k = -i kx phi_k
+        a[3 * nx + s] = make_double2(pi * kyv, -pr * kyv);  // Ey_k = -i ky phi_k
+    }
+    __syncthreads();
+    lines_inverse(a + nx, 3, nx, lgx, tw);
+    const double inv_n = 1.0 / static_cast<double>(nx);  // exact: nx is a power of two
+    for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
+        const int q = idx / nx;
+        const int x = idx - q * nx;
+        const double2 v = a[nx + idx];
+        T[(static_cast<int64_t>(q) * nx + x) * nky + ky_i] = make_double2(v.x * inv_n, v.y * inv_n);
+    }
+}
+
+// rows x0 .. x0 + rows - 1: irfft along y of phi_k, Ex_k, Ey_k (bins k <=
+// ny/2; the imaginary parts of the DC and Nyquist bins dropped, as irfft
+// does).  Ex and Ey are real, so one inverse FFT of the Hermitian extension
+// of Ex_k + i Ey_k gives Ex + i Ey; phi takes a transform of its own.
+__global__ __launch_bounds__(256) void p2d_fft_rows_inv(const double2* __restrict__ T, int nx, int ny, int lgy,
+                                                        int rows, double* __restrict__ phi, double* __restrict__ Ex,
+                                                        double* __restrict__ Ey) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* tw = reinterpret_cast<double2*>(smem_raw);  // [ny - 1]
+    double2* a = tw + ny;                                  // [rows][2][ny]: Ex + i Ey, then phi
+    const int nky = ny / 2 + 1;
+    const int half = ny / 2;
+    const int x0 = blockIdx.x * rows;
+    fill_stage_twiddles(tw, ny);
+    for (int idx = threadIdx.x; idx < rows * nky; idx += blockDim.x) {
+        const int r = idx / nky;
+        const int k = idx - r * nky;
+        const int x = x0 + r;
+        const double2 P = T[(static_cast<int64_t>(0) * nx + x) * nky + k];
+        const double2 E = T[(static_cast<int64_t>(1) * nx + x) * nky + k];
+        const double2 Y = T[(static_cast<int64_t>(2) * nx + x) * nky + k];
+        double2* z = a + (2 * r) * ny;
+        double2* w = z + ny;
+        if (k == 0 || k == half) {
+            z[brev_lg(k, lgy)] = make_double2(E.x, Y.x);
+            w[brev_lg(k, lgy)] = make_double2(P.x, 0.0);
+        } else {
+            z[brev_lg(k, lgy)] = make_double2(E.x - Y.y, E.y + Y.x);         // E_k + i Y_k
+            z[brev_lg(ny - k, lgy)] = make_double2(E.x + Y.y, Y.x - E.y);    // conj(E_k) + i conj(Y_k)
+            w[brev_lg(k, lgy)] = P;
+            w[brev_lg(ny - k, lgy)] = make_double2(P.x, -P.y);
+        }
+    }
+    __syncthreads();
+    lines_inverse(a, 2 * rows, ny, lgy, tw);
+    const double inv_n = 1.0 / static_cast<double>(ny);  // exact: ny is a power of two
+    for (int idx = threadIdx.x; idx < rows * ny; idx += blockDim.x) {
+        const int r = idx / ny;
+        const int y = idx - r * ny;
+        const int64_t o = static_cast<int64_t>(x0 + r) * ny + y;
+        const double2 z = a[(2 * r) * ny + y];
+        if (Ex) Ex[o] = z.x * inv_n;
+        if (Ey) Ey[o] = z.y * inv_n;
+        if (phi) phi[o] = a[(2 * r + 1) * ny + y].x * inv_n;
+    }
+}
+
 __global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -551,6 +783,8 @@ size_t poisson2d_ws(int64_t nx, int64_t ny) {
     return static_cast<size_t>(4) * nx * (ny / 2 + 1) * sizeof(double2);
 }
 
+static bool fft2d_enabled() { return knob(VP_KNOB_POISSON_DFT) != 1; }
+
 int poisson2d_from(const double* rho, const double2* partial, int nparts, double dvx, double dvy, int64_t nx,
                    int64_t ny, const double* kx, const double* ky, const double* inv_ksq, double* Ex, double* Ey,
                    double* delta_rho, double* phi, void* ws, size_t ws_bytes, hipStream_t s,
@@ -562,6 +796,26 @@ int poisson2d_from(const double* rho, const double2* partial, int nparts, double
     const int nky = nyi / 2 + 1;
     double2* R = static_cast<double2*>(ws);
     double2* T = R + static_cast<int64_t>(nx) * nky;
+    auto pow2 = [](int n) { return n >= 8 && (n & (n - 1)) == 0; };
+    if (pow2(nxi) && pow2(nyi) && fft2d_enabled()) {
+        const int lgx = 31 - __builtin_clz(static_cast<unsigned>(nxi));
+        const int lgy = 31 - __builtin_clz(static_cast<unsigned>(nyi));
+        // rows per workgroup: 256 / ny (at least 1, at most nx): nx is a multiple
+        int rows = 256 / nyi;
+        rows = rows < 1 ? 1 : (rows > nxi ? nxi : rows);
+        hipLaunchKernelGGL(p2d_fft_rows_fwd, dim3(nxi / rows), dim3(256), (ny + rows * ny) * sizeof(double2), s, rho,
+                           partial, nparts, dvx, dvy, nxi, nyi, lgy, rows, rho_shift, delta_rho, R);
+        VP_CHECK_LAUNCH();
+        hipLaunchKernelGGL(p2d_fft_cols, dim3(nky), dim3(256), 5 * nx * sizeof(double2), s, R, nxi, nyi, lgx, kx, ky,
+                           inv_ksq, T);
+        VP_CHECK_LAUNCH();
+        int rows_i = 256 / nyi;
+        rows_i = rows_i < 1 ? 1 : (rows_i > nxi ? nxi : rows_i);
+        hipLaunchKernelGGL(p2d_fft_rows_inv, dim3(nxi / rows_i), dim3(256), (ny + 2 * rows_i * ny) * sizeof(double2),
+                           s, T, nxi, nyi, lgy, rows_i, phi, Ex, Ey);
+        VP_CHECK_LAUNCH();
+        return VP_OK;
+    }
     hipLaunchKernelGGL(p2d_rows_fwd, dim3(nxi), dim3(256), ny * (sizeof(double2) + sizeof(double)), s, rho, partial,
                        nparts, dvx, dvy, nxi, nyi, rho_shift, delta_rho, R);
     VP_CHECK_LAUNCH();

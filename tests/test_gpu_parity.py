@@ -162,9 +162,15 @@ def test_rho_and_poisson_1d(device, nx):
     assert rel(phi.cpu().numpy(), phi_ref) <= 1e-13
 
 
-@pytest.mark.parametrize("n", [(16, 16), (64, 32), (128, 128), (24, 20)])
+@pytest.mark.parametrize("n", [(16, 16), (64, 32), (128, 128), (24, 20), (8, 8), (8, 512), (512, 16), (256, 256),
+                               (32, 128), (128, 64), (512, 512), (96, 128)])
 def test_poisson_2d(device, n):
+    """Power-of-two grids run radix-2 FFTs (rows of 8..512 points, several
+    rows per workgroup, Ex + i Ey through one inverse), other grids the
+    exact-twiddle DFTs; both against the oracle (numpy.fft), and the FFT
+    path against the DFT kernels forced by the knob."""
     v2 = _v2()
+    from vlasov_amd import _lib
     Nx, Ny = n
     g = vo.grid_2d(Nx, Ny, 4, 4, 4 * math.pi, 3 * math.pi, 6.0, 6.0)
     rng = np.random.default_rng(5)
@@ -175,6 +181,17 @@ def test_poisson_2d(device, n):
     names = ["Ex", "Ey", "delta_rho", "phi"]
     for name, o, r in zip(names, out, ref):
         assert rel(o.cpu().numpy(), r) <= 1e-13, (name, rel(o.cpu().numpy(), r))
+    lib = _lib.lib()
+    old = lib.vp_tune_get(_lib.KNOBS["poisson_dft"])
+    try:
+        lib.vp_tune_set(_lib.KNOBS["poisson_dft"], 1)
+        dft = v2.compute_poisson_real(t(rho), t(g.kx), t(g.ky), t(g.inv_ksq))
+    finally:
+        lib.vp_tune_set(_lib.KNOBS["poisson_dft"], old)
+    for name, o, r in zip(names, out, dft):
+        assert rel(o.cpu().numpy(), r.cpu().numpy()) <= 1e-13, name
+        if name == "delta_rho":
+            assert torch.equal(o, r)
 
 
 def test_compute_rho_2d(device):
