@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true", help="skip training the Fourier-operator surrogate")
     ap.add_argument("--no-reference-configs", action="store_true",
                     help="skip the per-step times on the configurations the reference published")
-    ap.add_argument("--surrogate-iters", type=int, default=36000)
+    ap.add_argument("--surrogate-iters", type=int, default=18000, help="training iterations per member network")
     ap.add_argument("--summarize-counters", metavar="DIR",
                     help="write profiles/traffic_2d2v_fp64.json from the four rocprofv3 --pmc runs under DIR "
                          "and exit (no GPU needed)")
@@ -229,32 +229,38 @@ def ensemble_bench(steps: int, n: int = 512, seed: int = 0, sample: int = 8, sam
     }
 
 
-def surrogate_bench(iters: int, ntrain: int = 20000, ntest: int = 1000, steps: int = 500) -> dict:
+def surrogate_bench(iters: int, ntrain: int = 20000, ncal: int = 2000, ntest: int = 1000, steps: int = 500,
+                    tol: float = 0.02, quantile: float = 0.95) -> dict:
     """Fourier-operator surrogate of log W_E(t): data from the batched solver
-    (128 x 256 grid, 500 steps), trained on the GPU kernels, evaluated on
-    held-out parameters against the solver."""
+    (128 x 256 grid, 500 steps), two member networks trained on the GPU
+    kernels (`iters` iterations each), routing calibrated on held-out solver
+    runs, then evaluated on other held-out runs against the solver: accuracy
+    of the mean prediction, and the routed query at `tol` (served fraction,
+    error of the served answers, throughput against solving every query)."""
     import torch
 
     from vlasov_amd import surrogate as sg
 
-    ptr, pte = sg.sample_params(ntrain, 1), sg.sample_params(ntest, 2)
+    ptr, pcal, pte = sg.sample_params(ntrain, 1), sg.sample_params(ncal, 3), sg.sample_params(ntest, 2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    Wtr = sg.solve(ptr, steps)
+    Wtr, Mtr = sg.solve(ptr, steps, return_moments=True)
+    Wcal, Mcal = sg.solve(pcal, steps, return_moments=True)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
     Wte = sg.solve(pte, steps)
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
-    sur = sg.Surrogate(steps)
+    sur = sg.Surrogate(steps, members=2)
     t0 = time.perf_counter()
-    _progress(f"surrogate: data {t_data:.1f} s; training {iters} iterations")
-    losses = sur.fit(ptr, Wtr, iters=iters, uniform=0.1,
-                     log=lambda x: _progress(f"surrogate: iteration {x[0]} loss {x[1]:.3g}") if x[0] % 4000 == 0
-                     else None)
+    _progress(f"surrogate: data {t_data:.1f} s; training 2 x {iters} iterations")
+    losses = sur.fit(ptr, Wtr, Mtr, iters=iters, uniform=0.1,
+                     log=lambda x: _progress(f"surrogate: member {x[0]} iteration {x[1]} loss {x[2]:.3g}")
+                     if x[1] % 3000 == 0 else None)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    sur.calibrate(pcal, Wcal, Mcal)
     sur.predict(pte[:8])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -263,18 +269,43 @@ def surrogate_bench(iters: int, ntrain: int = 20000, ntest: int = 1000, steps: i
     t_pred = time.perf_counter() - t0
     e = sg.rel_l2(Wp, Wte).cpu().numpy()
     elog = (torch.log(Wp) - torch.log(Wte)).abs().mean(dim=1).cpu().numpy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Wq, info = sur.query(pte, tol=tol, quantile=quantile)
+    torch.cuda.synchronize()
+    t_query = time.perf_counter() - t0
+    served = info.pop("served_mask")
+    eq = sg.rel_l2(Wq, Wte).cpu().numpy()
+    es = eq[served]
     return {
-        "model": "1D Fourier neural operator over time: 4 layers, width 64, 32 modes, BF16 MFMA / FP32 accumulate",
+        "model": "1D Fourier neural operator over time, 2 members: 4 layers, width 64, 32 modes, "
+                 "BF16 MFMA / FP32 accumulate",
+        "input": "(k, amplitude, drift, sigma) + f0 on the 128 x 256 grid, encoded by vp_nn_f0_moments "
+                 "(density mode amplitude, <v^2>, <v^4>/<v^2>^2)",
         "loss": "log W_E error weighted 0.1 / T + 0.9 W_t^2 / sum W^2 per run (the relative L2 error to first order)",
-        "train_runs": ntrain, "test_runs": ntest, "steps": steps, "grid": [128, 256],
-        "data_seconds": t_data, "train_iters": iters, "train_seconds": t_train,
-        "final_loss": losses[-1][1],
+        "train_runs": ntrain, "calibration_runs": ncal, "test_runs": ntest, "steps": steps, "grid": [128, 256],
+        "data_seconds": t_data, "train_iters_per_member": iters, "train_seconds": t_train,
+        "final_loss": [l[2] for l in losses if l[1] == iters - 1],
         "test_rel_l2_median": float(np.median(e)), "test_rel_l2_p90": float(np.percentile(e, 90)),
         "test_frac_within_2pct": float(np.mean(e <= 0.02)), "test_frac_within_10pct": float(np.mean(e <= 0.10)),
         "test_mean_abs_log_err_median": float(np.median(elog)),
         "predict_queries_per_s": ntest / t_pred, "solver_queries_per_s": ntest / t_solve,
         "speedup_vs_solver": t_solve / t_pred,
-        "fallback_rule": "judged relative L2 error of the history > 2% -> the request is answered by the solver",
+        "routing": {
+            "rule": f"served when the members' disagreement <= the largest value at which <= {1 - quantile:.0%} of "
+                    f"the {ncal} calibration runs miss the tolerance; else the solver; {5}% of served queries "
+                    "judged by the solver",
+            "tol": tol, "quantile": quantile, "threshold": info["threshold"],
+            "served_frac": float(served.mean()),
+            "served_frac_within_tol": float(np.mean(es <= tol)) if len(es) else None,
+            "served_err_median": float(np.median(es)) if len(es) else None,
+            "served_err_p95": float(np.percentile(es, 95)) if len(es) else None,
+            "served_err_max": float(es.max()) if len(es) else None,
+            "judged": info["judged"], "judged_misses": info["judged_misses"], "judge_max_err": info["judge_max_err"],
+            "fallback": info["fallback"],
+            "query_seconds": t_query, "solver_seconds": t_solve,
+            "throughput_gain_vs_solver": t_solve / t_query,
+        },
     }
 
 

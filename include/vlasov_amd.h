@@ -390,6 +390,14 @@ int vp_nn_wmse(const float* pred, const float* target, const float* w, float* gr
 /* Adam with bias correction at step >= 1 (float32 master weights) */
 int vp_nn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                float eps, int step, vp_stream_t stream);
+/* Encoding of the surrogate's input f0 [n][nx][nv] (float64 device arrays)
+ * on each run's grid v_j = -vmax[r] + j dv, dv = 2 vmax[r]/(nv - 1):
+ * out[r][6] = ((1/nx) sum_i rho_i, (1/nx) sum_i rho_i cos(2 pi i/nx),
+ * -(1/nx) sum_i rho_i sin(2 pi i/nx), M0, M2, M4) with rho_i = sum_j f_ij dv
+ * and M_p = sum_j fbar_j v_j^p dv, fbar_j = (1/nx) sum_i f_ij (float64,
+ * compensated sums). */
+int vp_nn_f0_moments(const double* f0, int n, int nx, int nv, const double* vmax, double* out,
+                     vp_stream_t stream);
 
 /* ---------------------------------------------------------------------
  * Launch-shape knobs of the fused kernels (no reference counterpart).

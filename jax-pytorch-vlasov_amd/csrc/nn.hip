@@ -237,6 +237,62 @@ __global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, 
     }
 }
 
+// Encoding of the surrogate's initial condition f0 [n][nx][nv] (float64):
+// one workgroup per run reduces its f0 on the run's own grid (x_i = i L/nx,
+// v_j = -Vmax + j dv, dv = 2 Vmax/(nv - 1); the mode's phase depends on i
+// only, so L is not needed) to
+//   out[r][0..2] = (1/nx) sum_i rho(x_i) (1, cos(2 pi i/nx), sin(2 pi i/nx)),
+//                  rho(x_i) = sum_j f(x_i, v_j) dv   (density and its first
+//                  spatial Fourier mode, times 1/nx; the phase has e^{-i})
+//   out[r][3..5] = sum_j fbar(v_j) v_j^p dv, p = 0, 2, 4,
+//                  fbar = (1/nx) sum_i f(x_i, .)      (velocity moments)
+// Thread j owns velocity columns j, j + 256, ...; compensated sums.
+__global__ __launch_bounds__(256) void f0_moments_kernel(const double* __restrict__ f0, int nx, int nv,
+                                                         const double* __restrict__ vmax, double* __restrict__ out) {
+    __shared__ double scratch[2 * 256 / kWave];
+    const int r = blockIdx.x;
+    const double* f = f0 + static_cast<int64_t>(r) * nx * nv;
+    const double V = vmax[r];
+    const double dv = 2.0 * V / static_cast<double>(nv - 1);
+    KSum a0{0, 0}, ac{0, 0}, as{0, 0}, m0{0, 0}, m2{0, 0}, m4{0, 0};
+    for (int j = threadIdx.x; j < nv; j += blockDim.x) {
+        KSum c0{0, 0}, c1{0, 0}, s1{0, 0};
+        for (int i = 0; i < nx; ++i) {
+            const double v = f[static_cast<int64_t>(i) * nv + j];
+            double sn, cs;
+            sincospi(2.0 * static_cast<double>(i) / static_cast<double>(nx), &sn, &cs);
+            ksum_add(c0, v);
+            ksum_add(c1, v * cs);
+            ksum_add(s1, -v * sn);
+        }
+        const double col = c0.s + c0.c;  // sum over x of f(., v_j)
+        ksum_add(a0, col * dv);
+        ksum_add(ac, (c1.s + c1.c) * dv);
+        ksum_add(as, (s1.s + s1.c) * dv);
+        const double vj = -V + static_cast<double>(j) * dv;
+        const double fb = col / static_cast<double>(nx);
+        ksum_add(m0, fb * dv);
+        ksum_add(m2, fb * vj * vj * dv);
+        ksum_add(m4, fb * (vj * vj) * (vj * vj) * dv);
+    }
+    const double inv = 1.0 / static_cast<double>(nx);
+    const double r0 = block_ksum(a0, scratch) * inv;
+    const double rc = block_ksum(ac, scratch) * inv;
+    const double rs = block_ksum(as, scratch) * inv;
+    const double q0 = block_ksum(m0, scratch);
+    const double q2 = block_ksum(m2, scratch);
+    const double q4 = block_ksum(m4, scratch);
+    if (threadIdx.x == 0) {
+        double* o = out + 6 * static_cast<int64_t>(r);
+        o[0] = r0;
+        o[1] = rc;
+        o[2] = rs;
+        o[3] = q0;
+        o[4] = q2;
+        o[5] = q4;
+    }
+}
+
 static dim3 grid1d(int64_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); }
 
 }  // namespace vp
@@ -262,6 +318,16 @@ extern "C" int vp_nn_gemm(int batch, int M, int N, int K, float alpha, const flo
     const dim3 grid(tn, tm, batch * splitk);
     hipLaunchKernelGGL(vp::gemm_bf16_kernel, grid, dim3(256), 0, s, M, N, K, splitk, kc, alpha, A, sAb, sAi, sAk, B,
                        sBb, sBk, sBj, beta, C, sCb, sCi, sCj);
+    VP_CHECK_LAUNCH();
+    return VP_OK;
+}
+
+extern "C" int vp_nn_f0_moments(const double* f0, int n, int nx, int nv, const double* vmax, double* out,
+                                vp_stream_t stream) {
+    VP_RETURN_IF(!f0 || !vmax || !out, VP_ERR_ARG);
+    VP_RETURN_IF(n < 1 || nx < 1 || nv < 2 || n > 0x7fffffff / 2, VP_ERR_SHAPE);
+    hipLaunchKernelGGL(vp::f0_moments_kernel, dim3(n), dim3(256), 0, static_cast<hipStream_t>(stream), f0, nx, nv,
+                       vmax, out);
     VP_CHECK_LAUNCH();
     return VP_OK;
 }

@@ -85,6 +85,7 @@ _SIGS = {
     "vp_ens_run_f64": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_int, c_vp, c_vp, c_sz, c_vp]),
     "vp_nn_gemm": (c_int, [c_int, c_int, c_int, c_int, ctypes.c_float, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64,
                            c_i64, ctypes.c_float, c_vp, c_i64, c_i64, c_i64, c_vp]),
+    "vp_nn_f0_moments": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "vp_nn_gelu": (c_int, [c_vp, c_vp, c_i64, c_vp]),
     "vp_nn_gelu_bwd": (c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "vp_nn_bias_add": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp]),

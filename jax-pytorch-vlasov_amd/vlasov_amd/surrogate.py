@@ -4,11 +4,15 @@ accumulation, GELU, bias, MSE and Adam kernels; torch only holds memory).
 
 No reference counterpart: the reference has no learned model.  The network
 (a 1D Fourier neural operator along time) maps the initial-condition
-parameters (k, amplitude, drift, sigma) of a 1D1V run on a fixed grid to
-log W_E(t) at every step; it is trained only on the ensemble solver's
-output, and the solver stays the fallback and the judge of its error:
+parameters (k, amplitude, drift, sigma) of a 1D1V run plus an encoding of
+its initial distribution f0 on the 128 x 256 grid (density Fourier mode and
+velocity moments, ``vp_nn_f0_moments``) to log W_E(t) at every step; it is
+trained only on the ensemble solver's output, and the solver stays the
+fallback and the judge of its error.  Two independently initialised
+networks are trained; their disagreement, calibrated on held-out solver
+runs, decides per query whether the network's answer is served:
 
-    H0 = X0 P + bP                                  X0: [B, T, 6] features
+    H0 = X0 P + bP                                  X0: [B, T, 9] features
     for l < L:  Z = G(R_l(F H)) + H W_l + b_l,      H = gelu(Z) (l < L-1)
     y = gelu(H Q1 + bQ1) Q2 + bQ2                   y: [B, T]
 
@@ -25,7 +29,9 @@ import torch
 
 from ._lib import check, lib, stream_handle
 
-CIN = 6  # features: t/T, k, log10(amplitude), drift, sigma, 1
+# features: t/T; k, log10(amplitude), drift, sigma; from f0: log10 of the
+# density mode amplitude, <v^2>, <v^4>/<v^2>^2; 1
+CIN = 9
 
 
 def _dft_mats(T: int, M: int, device):
@@ -262,41 +268,96 @@ def in_box(params: np.ndarray) -> np.ndarray:
     return np.all((x >= BOX[:, 0] - 1e-12) & (x <= BOX[:, 1] + 1e-12), axis=1)
 
 
+def f0_batch(params: np.ndarray, nx: int = 128, nv: int = 256, device="cuda") -> torch.Tensor:
+    """Initial distributions [n, nx, nv] (float64, device) of the runs:
+    f0 = (1 + A cos(2 pi i / nx)) 1/2 [M_s(v - u) + M_s(v + u)] at x_i = i L / nx
+    (L = 2 pi / k), v_j = -V + j 2V / (nv - 1), V = 6 s + u, M_s the
+    Maxwellian of width s."""
+    p = torch.as_tensor(np.asarray(params, dtype=np.float64), device=device)
+    A, u, sg = p[:, 1:2], p[:, 2:3], p[:, 3:4]
+    V = 6.0 * sg + u
+    v = -V + torch.arange(nv, dtype=torch.float64, device=device)[None, :] * (2.0 * V / (nv - 1))
+    fv = (torch.exp(-0.5 * ((v - u) / sg) ** 2) + torch.exp(-0.5 * ((v + u) / sg) ** 2)) / (
+        2.0 * sg * math.sqrt(2.0 * math.pi))
+    ang = torch.cos(2.0 * math.pi * torch.arange(nx, dtype=torch.float64, device=device) / nx)[None, :]
+    return ((1.0 + A * ang)[:, :, None] * fv[:, None, :]).contiguous()
+
+
+def vmax_of(params: np.ndarray) -> np.ndarray:
+    """V_max = 6 s + u of each run."""
+    p = np.asarray(params, dtype=np.float64)
+    return 6.0 * p[:, 3] + p[:, 2]
+
+
+def encode_f0(f0: torch.Tensor, vmax) -> torch.Tensor:
+    """vp_nn_f0_moments on f0 [n, nx, nv]: [n, 6] (float64, device)."""
+    n, nx, nv = f0.shape
+    vm = torch.as_tensor(np.asarray(vmax, dtype=np.float64), device=f0.device).contiguous()
+    f0 = f0.contiguous()
+    out = torch.empty((n, 6), dtype=torch.float64, device=f0.device)
+    check(lib().vp_nn_f0_moments(f0.data_ptr(), n, nx, nv, vm.data_ptr(), out.data_ptr(),
+                                 stream_handle(f0.device)), "vp_nn_f0_moments")
+    return out
+
+
 def solve(params: np.ndarray, steps: int = 500, nx: int = 128, nv: int = 256, dt: float = 0.1,
-          batch: int = 2048, device="cuda") -> torch.Tensor:
+          batch: int = 2048, device="cuda", return_moments: bool = False):
     """Field-energy histories [n, steps] (float64, device) of 1D1V runs on an
-    nx x nv grid, f0 = (1 + A cos(k x)) 1/2 [M_s(v - u) + M_s(v + u)] with the
-    Maxwellian M_s of width s, L = 2 pi / k, V_max = 6 s + u, run by the
-    batched ensemble solver."""
+    nx x nv grid from f0_batch(params) (L = 2 pi / k, V_max = 6 s + u), run by
+    the batched ensemble solver; with return_moments, also the encodings of
+    their f0 (encode_f0)."""
     from .ensemble import Ensemble
 
-    out = []
+    params = np.asarray(params, dtype=np.float64)
+    out, mom = [], []
     for i0 in range(0, len(params), batch):
         chunk = params[i0:i0 + batch]
-        specs = [{"nx": nx, "nv": nv, "L": 2.0 * math.pi / k, "V_max": 6.0 * sig + u,
-                  "f0": torch.zeros((nx, nv), dtype=torch.float64, device=device)} for k, amp, u, sig in chunk]
+        f0 = f0_batch(chunk, nx, nv, device)
+        if return_moments:
+            mom.append(encode_f0(f0, vmax_of(chunk)))
+        specs = [{"nx": nx, "nv": nv, "L": 2.0 * math.pi / k, "V_max": 6.0 * sig + u, "f0": f0[i]}
+                 for i, (k, amp, u, sig) in enumerate(chunk)]
         ens = Ensemble(specs, dt, physics_mode=True, device=device)
-        for i, (k, amp, u, sig) in enumerate(chunk):
-            g = ens.grids[i]
-            fv = (torch.exp(-0.5 * ((g.v - u) / sig) ** 2) + torch.exp(-0.5 * ((g.v + u) / sig) ** 2)) / (
-                2.0 * sig * math.sqrt(2.0 * math.pi))
-            ens.state(i).copy_((1.0 + amp * torch.cos(k * g.x))[:, None] * fv[None, :])
         out.append(ens.run(steps).T.contiguous())
-        del ens
-    return torch.cat(out)
+        del ens, f0
+    W = torch.cat(out)
+    return (W, torch.cat(mom)) if return_moments else W
 
 
-def features(params: np.ndarray, T: int, device="cuda") -> torch.Tensor:
-    """Network input [n, T, 6]: t / T and the four parameters scaled to [-1, 1]
-    over the box (amplitude in log10), and a constant 1."""
+def f0_features(moments: torch.Tensor) -> torch.Tensor:
+    """[n, 3] from the f0 encoding: log10 of the density mode amplitude
+    2 |rho_1| / rho_0, <v^2> = M2 / M0 and the kurtosis <v^4> / <v^2>^2
+    (3 for one Maxwellian, lower for two separated beams)."""
+    m = moments.double()
+    A = 2.0 * torch.hypot(m[:, 1], m[:, 2]) / m[:, 0]
+    m2 = m[:, 4] / m[:, 3]
+    m4 = m[:, 5] / m[:, 3]
+    return torch.stack([torch.log10(torch.clamp(A, min=1e-300)), m2, m4 / (m2 * m2)], dim=1)
+
+
+# ranges of the f0 features over the parameter box (log10 A; <v^2> = s^2 + u^2; kurtosis)
+F0_BOX = np.array([[math.log10(1e-3), math.log10(0.5)], [0.64, 10.44], [1.0, 3.0]])
+
+
+def features(params: np.ndarray, T: int, moments: torch.Tensor | None = None, device="cuda") -> torch.Tensor:
+    """Network input [n, T, 9]: t / T; the four parameters scaled to [-1, 1]
+    over the box (amplitude in log10); the three f0 features (f0_features of
+    `moments`, the f0 encodings; computed from f0_batch(params) when None)
+    scaled over their ranges; a constant 1."""
     x = np.asarray(params, dtype=np.float64).copy()
     x[:, 1] = np.log10(x[:, 1])
     z = 2.0 * (x - BOX[:, 0]) / (BOX[:, 1] - BOX[:, 0]) - 1.0
     n = len(x)
+    if moments is None:
+        moments = torch.cat([encode_f0(f0_batch(params[i:i + 2048], device=device), vmax_of(params[i:i + 2048]))
+                             for i in range(0, n, 2048)])
+    ff = f0_features(moments).cpu().numpy()
+    zf = 2.0 * (ff - F0_BOX[:, 0]) / (F0_BOX[:, 1] - F0_BOX[:, 0]) - 1.0
     X = np.empty((n, T, CIN), dtype=np.float32)
     X[:, :, 0] = (np.arange(T) / T)[None, :]
     X[:, :, 1:5] = z[:, None, :]
-    X[:, :, 5] = 1.0
+    X[:, :, 5:8] = zf[:, None, :]
+    X[:, :, 8] = 1.0
     return torch.from_numpy(X).to(device)
 
 
@@ -306,15 +367,20 @@ def rel_l2(W_hat: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
 
 
 class Surrogate:
-    """FNO surrogate of log W_E(t), trained on the solver's histories."""
+    """Two-member FNO surrogate of log W_E(t), trained on the solver's
+    histories, with per-query routing by the members' disagreement."""
 
     def __init__(self, steps: int = 500, width: int = 64, modes: int = 32, hidden: int = 128, layers: int = 4,
-                 seed: int = 0, device="cuda"):
+                 members: int = 2, seed: int = 0, device="cuda"):
         self.steps = steps
         self.device = torch.device(device)
-        self.net = FNO(steps, width, modes, hidden, layers, seed=seed, device=self.device)
+        self.nets = [FNO(steps, width, modes, hidden, layers, seed=seed + 7919 * i, device=self.device)
+                     for i in range(members)]
+        self.net = self.nets[0]
         self.mu = 0.0
         self.sd = 1.0
+        self.cal_spread = None  # calibration runs: disagreement, sorted
+        self.cal_err = None     # and the matching judged errors
 
     FLOOR = 1e-6  # log target floor, relative to each run's maximum
 
@@ -322,15 +388,17 @@ class Surrogate:
         floor = W.max(dim=1, keepdim=True).values * self.FLOOR
         return torch.log(torch.maximum(W, floor))
 
-    def fit(self, params: np.ndarray, W: torch.Tensor, iters: int = 16000, batch: int = 128, lr: float = 3e-3,
-            seed: int = 0, log=None, uniform: float = 0.1) -> list:
+    def fit(self, params: np.ndarray, W: torch.Tensor, moments: torch.Tensor | None = None, iters: int = 16000,
+            batch: int = 128, lr: float = 3e-3, seed: int = 0, log=None, uniform: float = 0.1) -> list:
         """Adam on the weighted mean squared error of the normalized log
-        history; cosine learning-rate decay.  Per run the weight of time t
-        is uniform / T + (1 - uniform) W_t^2 / sum W^2: the second part
-        makes the loss the squared relative L2 error of the history (to
-        first order), the metric the solver judges the surrogate by.
-        Returns (iteration, loss) every 50 iterations; `log` (a callable)
-        receives them as they come."""
+        history, cosine learning-rate decay, `iters` iterations per member
+        (each member its own initialisation and minibatch order).  Per run
+        the weight of time t is uniform / T + (1 - uniform) W_t^2 / sum W^2:
+        the second part makes the loss the squared relative L2 error of the
+        history (to first order), the metric the solver judges the
+        surrogate by.  `moments`: the f0 encodings of the training runs
+        (computed when None).  Returns (member, iteration, loss) every 50
+        iterations; `log` (a callable) receives them as they come."""
         logW = self._log_target(W)
         self.mu = float(logW.mean())
         self.sd = float(logW.std())
@@ -338,68 +406,124 @@ class Surrogate:
         W2 = W.double() ** 2
         Wt = (uniform / self.steps + (1.0 - uniform) * W2 / W2.sum(dim=1, keepdim=True)) / batch
         Wt = Wt.float().contiguous()
-        X = features(params, self.steps, self.device)
+        X = features(params, self.steps, moments, self.device)
         n = len(params)
-        gen = torch.Generator(device="cpu").manual_seed(seed)
         dy = torch.empty((batch, self.steps), dtype=torch.float32, device=self.device)
         losses = []
-        for it in range(iters):
-            idx = torch.randint(0, n, (batch,), generator=gen).to(self.device)
-            xb = X.index_select(0, idx).contiguous()
-            yb = Y.index_select(0, idx).contiguous()
-            wb = Wt.index_select(0, idx).contiguous()
-            out = self.net.forward(xb)
-            sse = self.net.wmse(out, yb, wb, dy)
-            self.net.backward(dy)
-            lr_t = 1e-5 + 0.5 * (lr - 1e-5) * (1.0 + math.cos(math.pi * it / max(iters - 1, 1)))
-            self.net.adam(lr_t)
-            if it % 50 == 0 or it == iters - 1:
-                losses.append((it, float(sse)))
-                if log is not None:
-                    log(losses[-1])
+        for mi, net in enumerate(self.nets):
+            gen = torch.Generator(device="cpu").manual_seed(seed + 104729 * mi)
+            for it in range(iters):
+                idx = torch.randint(0, n, (batch,), generator=gen).to(self.device)
+                xb = X.index_select(0, idx).contiguous()
+                yb = Y.index_select(0, idx).contiguous()
+                wb = Wt.index_select(0, idx).contiguous()
+                out = net.forward(xb)
+                sse = net.wmse(out, yb, wb, dy)
+                net.backward(dy)
+                lr_t = 1e-5 + 0.5 * (lr - 1e-5) * (1.0 + math.cos(math.pi * it / max(iters - 1, 1)))
+                net.adam(lr_t)
+                if it % 50 == 0 or it == iters - 1:
+                    losses.append((mi, it, float(sse)))
+                    if log is not None:
+                        log(losses[-1])
         return losses
 
-    def predict(self, params: np.ndarray, batch: int = 512) -> torch.Tensor:
-        """W_E histories [n, steps] (float64, device)."""
-        X = features(params, self.steps, self.device)
+    def predict_members(self, params: np.ndarray, moments: torch.Tensor | None = None,
+                        batch: int = 512) -> torch.Tensor:
+        """log W_E histories of every member [members, n, steps] (float64)."""
+        X = features(params, self.steps, moments, self.device)
         outs = []
-        for i0 in range(0, len(params), batch):
-            y = self.net.forward(X[i0:i0 + batch].contiguous())
-            outs.append(torch.exp(y.double() * self.sd + self.mu))
-        return torch.cat(outs)
+        for net in self.nets:
+            rows = []
+            for i0 in range(0, len(params), batch):
+                y = net.forward(X[i0:i0 + batch].contiguous())
+                rows.append(y.double() * self.sd + self.mu)
+            outs.append(torch.cat(rows))
+        return torch.stack(outs)
 
-    def query(self, params: np.ndarray, tol: float = 0.02, judge_fraction: float = 0.05, seed: int = 0,
-              **solver_kw) -> tuple:
-        """Serve W_E histories with the solver as fallback and judge.
+    def predict(self, params: np.ndarray, moments: torch.Tensor | None = None, batch: int = 512,
+                with_spread: bool = False):
+        """W_E histories [n, steps] (float64, device): the exponential of the
+        members' mean log history.  with_spread: also each run's
+        disagreement, the largest relative L2 distance of a member's
+        history from the served one."""
+        logs = self.predict_members(params, moments, batch)
+        W = torch.exp(logs.mean(dim=0))
+        if not with_spread:
+            return W
+        spread = torch.stack([rel_l2(torch.exp(l), W) for l in logs]).max(dim=0).values
+        return W, spread
 
-        Parameters outside the training box go to the solver.  A random
-        `judge_fraction` of the rest (at least one) is also solved and
-        compared; if any judged run misses `tol` (relative L2 error of the
-        history) the whole request falls back to the solver.  Returns
-        (histories [n, steps], info)."""
+    def calibrate(self, params: np.ndarray, W: torch.Tensor, moments: torch.Tensor | None = None) -> None:
+        """Record disagreement and true error on held-out solver runs (not
+        used for training): the routing threshold of `query` comes from them."""
+        Wp, spread = self.predict(params, moments, with_spread=True)
+        err = rel_l2(Wp, W)
+        order = torch.argsort(spread)
+        self.cal_spread = spread[order].cpu().numpy()
+        self.cal_err = err[order].cpu().numpy()
+
+    def threshold(self, tol: float, quantile: float = 0.95, min_runs: int = 20) -> float:
+        """Largest disagreement s such that, over the calibration runs with
+        disagreement <= s (at least min_runs of them), the fraction whose
+        error exceeds tol is at most 1 - quantile (-inf without calibration
+        or when no such s exists)."""
+        if self.cal_spread is None:
+            return -math.inf
+        miss = np.cumsum(self.cal_err > tol) / np.arange(1, len(self.cal_err) + 1)
+        ok = np.nonzero((miss <= 1.0 - quantile) & (np.arange(1, len(miss) + 1) >= min_runs))[0]
+        return float(self.cal_spread[ok[-1]]) if len(ok) else -math.inf
+
+    def query(self, params: np.ndarray, tol: float = 0.02, quantile: float = 0.95, judge_fraction: float = 0.05,
+              seed: int = 0, **solver_kw) -> tuple:
+        """Serve W_E histories, each query routed on its own.
+
+        A query is answered by the network when its parameters lie in the
+        training box and the members' disagreement is at most
+        threshold(tol, quantile): on the held-out calibration runs, a
+        fraction `quantile` of the answers below that disagreement were
+        within `tol` (relative L2 error of the history).  Every other query
+        is answered by the solver.  A random `judge_fraction` of the served
+        queries (at least one) is also solved; those keep the solver's
+        answer.  If the judged misses exceed what the calibration allows
+        (more than (1 - quantile) n_judged + 2 sqrt((1 - quantile) n_judged)
+        + 1), the request lies outside the calibrated distribution and every
+        query goes to the solver.  Returns (histories [n, steps], info)."""
         params = np.asarray(params, dtype=np.float64)
         n = len(params)
         inside = in_box(params)
         W = torch.empty((n, self.steps), dtype=torch.float64, device=self.device)
-        info = {"n": n, "outside_box": int((~inside).sum()), "judged": 0, "judge_max_err": None,
-                "fallback": False}
+        thr = self.threshold(tol, quantile)
+        info = {"n": n, "outside_box": int((~inside).sum()), "threshold": thr, "served": 0, "solved": 0,
+                "judged": 0, "judged_misses": 0, "judge_max_err": None, "fallback": False}
+        to_solve = ~inside
+        judged = np.zeros(n, dtype=bool)
         idx_in = np.nonzero(inside)[0]
-        idx_out = np.nonzero(~inside)[0]
-        if len(idx_out):
-            W[torch.from_numpy(idx_out).to(self.device)] = solve(params[idx_out], self.steps, **solver_kw)
         if len(idx_in):
-            pred = self.predict(params[idx_in])
-            rng = np.random.default_rng(seed)
-            nj = max(1, int(round(judge_fraction * len(idx_in))))
-            jsel = rng.choice(len(idx_in), size=min(nj, len(idx_in)), replace=False)
-            Wj = solve(params[idx_in[jsel]], self.steps, **solver_kw)
-            err = rel_l2(pred[torch.from_numpy(jsel).to(self.device)], Wj)
-            info["judged"] = len(jsel)
-            info["judge_max_err"] = float(err.max())
-            if float(err.max()) > tol:
-                info["fallback"] = True
-                pred = solve(params[idx_in], self.steps, **solver_kw)
-            else:
-                pred[torch.from_numpy(jsel).to(self.device)] = Wj  # judged runs keep the solver's answer
+            pred, spread = self.predict(params[idx_in], with_spread=True)
+            sp = spread.cpu().numpy()
+            ok = sp <= thr
+            serve = idx_in[ok]
+            to_solve[idx_in[~ok]] = True
             W[torch.from_numpy(idx_in).to(self.device)] = pred
+            if len(serve):
+                rng = np.random.default_rng(seed)
+                nj = min(len(serve), max(1, int(round(judge_fraction * len(serve)))))
+                jidx = serve[np.sort(rng.choice(len(serve), size=nj, replace=False))]
+                Wj = solve(params[jidx], self.steps, **solver_kw)
+                err = rel_l2(W[torch.from_numpy(jidx).to(self.device)], Wj).cpu().numpy()
+                W[torch.from_numpy(jidx).to(self.device)] = Wj  # judged queries keep the solver's answer
+                judged[jidx] = True
+                misses = int((err > tol).sum())
+                info.update(judged=nj, judged_misses=misses, judge_max_err=float(err.max()))
+                expect = (1.0 - quantile) * nj
+                if misses > expect + 2.0 * math.sqrt(expect) + 1.0:
+                    info["fallback"] = True
+                    to_solve[serve] = True
+                    to_solve[jidx] = False
+        sidx = np.nonzero(to_solve)[0]
+        if len(sidx):
+            W[torch.from_numpy(sidx).to(self.device)] = solve(params[sidx], self.steps, **solver_kw)
+        served = ~(to_solve | judged)
+        info.update(served=int(served.sum()), solved=int(n - served.sum()), served_mask=served)
         return W, info

@@ -131,3 +131,42 @@ def mse(y, target):
     """Mean squared error and its gradient with respect to y."""
     d = y - target
     return float(np.mean(d * d)), 2.0 * d / d.size
+
+
+def f0_moments(f0, vmax):
+    """Encoding of the surrogate's input f0 [n, nx, nv] on the grids
+    v_j = -vmax[r] + j dv, dv = 2 vmax[r]/(nv - 1) (the quantities the
+    vp_nn_f0_moments kernel returns): per run
+    ((1/nx) sum_i rho_i, (1/nx) sum_i rho_i cos(2 pi i/nx),
+    -(1/nx) sum_i rho_i sin(2 pi i/nx), M0, M2, M4), rho_i = sum_j f_ij dv,
+    M_p = sum_j fbar_j v_j^p dv, fbar = (1/nx) sum_i f_ij.
+    Pinned by closed forms (tests/test_oracle_fno.py): for
+    f0 = (1 + A cos(2 pi i/nx)) 1/2 [M_s(v - u) + M_s(v + u)] on a wide box the
+    results are (1, A/2, 0, 1, s^2 + u^2, u^4 + 6 u^2 s^2 + 3 s^4)."""
+    f0 = np.asarray(f0, dtype=np.float64)
+    n, nx, nv = f0.shape
+    out = np.empty((n, 6))
+    ang = 2.0 * math.pi * np.arange(nx) / nx
+    for r in range(n):
+        V = float(vmax[r])
+        dv = 2.0 * V / (nv - 1)
+        v = -V + np.arange(nv) * dv
+        rho = f0[r].sum(axis=1) * dv
+        fbar = f0[r].sum(axis=0) / nx
+        out[r] = (rho.sum() / nx, (rho * np.cos(ang)).sum() / nx, -(rho * np.sin(ang)).sum() / nx,
+                  (fbar * dv).sum(), (fbar * v * v * dv).sum(), (fbar * v ** 4 * dv).sum())
+    return out
+
+
+def decode_f0_moments(m):
+    """(amplitude, drift u, width s) of a two-beam f0 from its encoding:
+    A = 2 |rho_1| / rho_0, u^4 = (3 M2'^2 - M4') / 2 and s^2 = M2' - u^2 with
+    M2' = M2 / M0, M4' = M4 / M0 (the moments of 1/2 [M_s(v - u) + M_s(v + u)]
+    are M2' = s^2 + u^2, M4' = u^4 + 6 u^2 s^2 + 3 s^4)."""
+    m = np.asarray(m, dtype=np.float64)
+    A = 2.0 * np.hypot(m[:, 1], m[:, 2]) / m[:, 0]
+    m2 = m[:, 4] / m[:, 3]
+    m4 = m[:, 5] / m[:, 3]
+    u = np.maximum(0.5 * (3.0 * m2 * m2 - m4), 0.0) ** 0.25
+    s = np.sqrt(np.maximum(m2 - u * u, 0.0))
+    return A, u, s

@@ -72,3 +72,16 @@ def test_noalloc_step_rejects_oversized_problems_without_gpu():
     assert rc == -5
     assert b"scratch" in L.vp_error_string(rc)
     assert L.vp_step_1d1v_f64_noalloc(8, 16, 0, 128, 0.1, 8, 8, 8, 0.1, 0.1, 1, 8, 8, 8, None) == -2
+
+
+def test_stream_argument_checks_without_gpu():
+    """vp_stream_f64 rejects bad kinds, odd sizes, misaligned pointers and
+    unroll factors before launching anything."""
+    from vlasov_amd import _lib
+    L = _lib.lib()
+    assert L.vp_stream_f64(3, 16, 32, 64, 1, 1, 1, None) == -1
+    assert L.vp_stream_f64(0, 16, 32, 63, 1, 1, 1, None) == -1
+    assert L.vp_stream_f64(0, 24, 32, 64, 1, 1, 1, None) == -1
+    assert L.vp_stream_f64(0, 16, 32, 64, 3, 1, 1, None) == -1
+    assert L.vp_stream_f64(0, 16, 32, 64, 1, 1, 0, None) == -1
+    assert L.vp_stream_f64(0, None, 32, 64, 1, 1, 1, None) == -1

@@ -82,3 +82,29 @@ def test_zero_spectral_weights_is_pointwise_mlp():
         H = fo.gelu(Z) if l < L - 1 else Z
     ref = (fo.gelu(H @ p["Q1"] + p["bQ1"]) @ p["Q2"] + p["bQ2"])[..., 0]
     np.testing.assert_allclose(y, ref, atol=1e-12)
+
+
+def test_f0_moments_closed_forms():
+    """Two-beam f0 on a wide velocity box: (rho_0, rho_1 cos, rho_1 sin, M0,
+    M2, M4) = (1, A/2, 0, 1, s^2 + u^2, u^4 + 6 u^2 s^2 + 3 s^4) to
+    quadrature accuracy, and decoding gives (A, u, s) back; a shifted phase
+    moves the mode into the sine part."""
+    rng = np.random.default_rng(0)
+    nx, nv = 32, 256
+    for _ in range(4):
+        A, u, s = rng.uniform(0.01, 0.5), rng.uniform(0.0, 3.0), rng.uniform(0.8, 1.2)
+        V = 10 * s + u
+        v = -V + np.arange(nv) * (2 * V / (nv - 1))
+        M = lambda w: np.exp(-0.5 * (w / s) ** 2) / (s * math.sqrt(2 * math.pi))
+        fv = 0.5 * (M(v - u) + M(v + u))
+        x = 2 * math.pi * np.arange(nx) / nx
+        f0 = ((1 + A * np.cos(x))[:, None] * fv[None, :])[None]
+        m = fo.f0_moments(f0, [V])[0]
+        want = (1.0, A / 2, 0.0, 1.0, s * s + u * u, u ** 4 + 6 * u * u * s * s + 3 * s ** 4)
+        np.testing.assert_allclose(m, want, rtol=1e-10, atol=1e-12)
+        A2, u2, s2 = fo.decode_f0_moments(m[None])
+        np.testing.assert_allclose([A2[0], s2[0]], [A, s], rtol=1e-8)
+        assert abs(u2[0] - u) < 1e-3 * max(u, 1.0) + 2e-3  # u^4 near 0 amplifies rounding for small u
+        f1 = ((1 + A * np.sin(x))[:, None] * fv[None, :])[None]
+        m1 = fo.f0_moments(f1, [V])[0]
+        np.testing.assert_allclose(m1[1:3], [0.0, -A / 2], atol=1e-12)
