@@ -218,8 +218,9 @@ def ensemble_bench(steps: int, n: int = 512, seed: int = 0, sample: int = 8, sam
         "cells": ens.cells,
         "cell_updates_per_s": ens.cells * steps / t,
         "energy_finite": finite,
-        "launch_shapes": {k: choice[k] for k in ("ens_cols", "ens_xthreads", "ens_rows")},
-        "graph": "one HIP graph of the whole run (3 launches per step)",
+        "launch_shapes": {k: choice[k] for k in ("ens_xtile", "ens_ktile", "ens_wgs")},
+        "graph": "one HIP graph of the whole run (a work-list set-up launch, then 3 launches per step: "
+                 "persistent X drift pair + rho, field solve, persistent kick)",
         "per_instance_loop": {"value": n / base_t, "unit": "solves/s",
                               "sample": f"{sample} instances across the size range x {sample_steps} steps, "
                                         "extrapolated per instance-step",

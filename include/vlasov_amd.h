@@ -333,9 +333,11 @@ int vp_moments_f32(const float* f, int64_t rows, int64_t len, const double* vx, 
  * reference runs one simulation per process).  Each instance follows
  * vlasov_step (P1:574-622) on its own f[nx][nv] (C order) inside one pool;
  * all instances advance together, three launches per step (X drift + rho
- * partial sums, field solve + field energy, V kick), the closing X(dt/2) of
- * a step fused with the opening X(dt/2) of the next.  Sweeps are
- * bit-identical to the single-instance path.
+ * partial sums, field solve + field energy, V kick; persistent sweep
+ * kernels walking work lists of equal-sized tiles that a set-up launch
+ * builds from `inst`), the closing X(dt/2) of a step fused with the opening
+ * X(dt/2) of the next.  Sweeps are bit-identical to the single-instance
+ * path.
  *
  * inst (device array, ninst entries): per instance the offsets of its f in
  * `pool` and of its tables in `tables` (v[nv] at v_off; inv_kx[nx/2+1] then
@@ -423,12 +425,12 @@ enum {
                                   streaming (non-temporal) hint, a bit mask:
                                   1 X drift, 2 Y drift, 4 kick; 8 = none
                                   (0 = 7)                                      */
-    VP_KNOB_ENS_COLS = 6,      /* ensemble X drift: 16 or 32 v columns per
-                                  workgroup (0 = 32)                           */
-    VP_KNOB_ENS_ROWS = 7,      /* ensemble kick: 2, 4, 8 or 16 x rows per
-                                  workgroup (0 = 4)                            */
-    VP_KNOB_ENS_XTHREADS = 8,  /* ensemble X drift: 256, 512 or 1024 threads
-                                  per workgroup (0 = 512)                      */
+    VP_KNOB_ENS_XTILE = 6,     /* ensemble X drift: elements per work item,
+                                  2048 or 4096 (0 = 4096)                      */
+    VP_KNOB_ENS_KTILE = 7,     /* ensemble kick: elements per work item, 2048
+                                  or 4096 (0 = 4096)                           */
+    VP_KNOB_ENS_WGS = 8,       /* ensemble sweeps: persistent workgroups per
+                                  CU, 1 or 2 (0 = 2)                           */
     VP_KNOB_SEG_ROWS = 9,      /* generic sweeps of strided lines longer than
                                   256: 16..128 rows per workgroup segment
                                   (0 = 112, or 64 / 32 when that gives under

@@ -19,6 +19,47 @@ __device__ __forceinline__ void weights_as(double alpha, T w[6]) {
     round_weights<T>(wd, w);
 }
 
+// x / d for a constant integer d in [3, 120] without the division sequence:
+// q = RN(x y) with y = RN(1/d), r = x - q d (exact by fma), RN(q + r y).
+// Equal to RN(x / d) for x = 0 and for every normal x whose quotient is
+// normal: q is within 2 ulp u of x/d, so r is a small multiple of u/2 and
+// exact, and q + r y differs from x/d by less than 2^-52 u, while x/d (x a
+// multiple of 2u, d >= 3) lies at least u/(2d) >= u/240 from every midpoint
+// of the double grid, so both round to the same double.  (Checked on 4e9
+// normal numerators for each divisor below.)
+__device__ __forceinline__ double div_const(double x, double d, double y) {
+    const double q = x * y;
+    const double r = fma(-q, d, x);
+    return fma(r, y, q);
+}
+
+// lagrange5_weights<double> with div_const, for alpha = 0 or alpha in
+// [2^-52, 1) (the fraction of a departure point with |t| >= 1): every
+// numerator (alpha^k times 1, 2, 5 or 7) is then 0 or normal (>= 2^-260)
+// and so is every quotient, so the weights are bit-identical to the
+// divisions of the reference's expression (vlasov1D1V.py:191-210).
+template <typename T>
+__device__ __forceinline__ void weights_binade(double a, T w[6]) {
+    constexpr double i3 = 1.0 / 3.0, i6 = 1.0 / 6.0, i12 = 1.0 / 12.0, i20 = 1.0 / 20.0, i24 = 1.0 / 24.0,
+                     i30 = 1.0 / 30.0, i120 = 1.0 / 120.0;
+    const double a2 = a * a;
+    const double a3 = a2 * a;
+    const double a4 = a3 * a;
+    const double a5 = a4 * a;
+    const double a3_24 = div_const(a3, 24.0, i24), a4_24 = div_const(a4, 24.0, i24);
+    const double a5_24 = div_const(a5, 24.0, i24), a5_12 = div_const(a5, 12.0, i12);
+    const double a5_120 = div_const(a5, 120.0, i120), a2_24 = div_const(a2, 24.0, i24);
+    const double a4_6 = div_const(a4, 6.0, i6), tw_a2_3 = div_const(2.0 * a2, 3.0, i3);
+    double wd[6];
+    wd[0] = div_const(a, 20.0, i20) - a2_24 - a3_24 + a4_24 - a5_120;
+    wd[1] = -(a / 2.0) + tw_a2_3 - a3_24 - a4_6 + a5_24;
+    wd[2] = 1.0 - div_const(a, 3.0, i3) - (5.0 * a2 / 4.0) + div_const(5.0 * a3, 12.0, i12) + (a4 / 4.0) - a5_12;
+    wd[3] = a + tw_a2_3 - div_const(7.0 * a3, 12.0, i12) - a4_6 + a5_12;
+    wd[4] = -(a / 4.0) - a2_24 + div_const(7.0 * a3, 24.0, i24) + a4_24 - a5_24;
+    wd[5] = div_const(a, 30.0, i30) - a3_24 + a5_120;
+    round_weights<T>(wd, w);
+}
+
 // Binade weight tables of the departure form.  For a line of displacement
 // disp = (vel dt)/dx and an integer index r, t = r - disp (rounded) gives
 // il = floor(t) and alpha = t - il.  When t lies in a binade [2^b, 2^(b+1))
@@ -35,15 +76,21 @@ __device__ __forceinline__ void binade_weights(double disp, int b, T w[6]) {
     const double lo = ldexp(1.0, b);
     const double hi = 2.0 * lo;
     const double c0 = ceil(lo + disp);
+    double ts = 0.0;
+    bool found = false;
+#pragma unroll
     for (int d = -1; d <= 2; ++d) {
         const double t = (c0 + d) - disp;  // the kernel's (double)r - disp for r = c0 + d
-        if (t >= lo && t < hi) {
-            weights_as<T>(t - floor(t), w);
-            return;
-        }
+        const bool in = !found && t >= lo && t < hi;
+        ts = in ? t : ts;
+        found = found || in;
     }
+    if (found) {
+        weights_binade<T>(ts - floor(ts), w);
+    } else {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = T(0);  // no index of this line reaches binade b
+        for (int k = 0; k < 6; ++k) w[k] = T(0);  // no index of this line reaches binade b
+    }
 }
 
 // The same for negative departure points, t in (-2^(b+1), -2^b]: the
@@ -54,15 +101,21 @@ __device__ __forceinline__ void binade_weights_neg(double disp, int b, T w[6]) {
     const double lo = ldexp(1.0, b);
     const double hi = 2.0 * lo;
     const double c0 = floor(disp - lo);
+    double ts = 0.0;
+    bool found = false;
+#pragma unroll
     for (int d = 1; d >= -2; --d) {
         const double t = (c0 + d) - disp;
-        if (-t >= lo && -t < hi) {
-            weights_as<T>(t - floor(t), w);
-            return;
-        }
+        const bool in = !found && -t >= lo && -t < hi;
+        ts = in ? t : ts;
+        found = found || in;
     }
+    if (found) {
+        weights_binade<T>(ts - floor(ts), w);
+    } else {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = T(0);
+        for (int k = 0; k < 6; ++k) w[k] = T(0);
+    }
 }
 
 // Signed binade code of a departure point: e - 1 for t >= 1, -e for t <= -1

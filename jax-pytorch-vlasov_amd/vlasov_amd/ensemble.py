@@ -133,9 +133,10 @@ class Ensemble:
         return g
 
     def autotune(self, steps: int = 2, reps: int = 3) -> dict:
-        """Pick the ensemble launch shapes (X-drift column chunk and threads,
-        kick rows per workgroup) by timing each candidate on a copy of the
-        pool; sets the library knobs and returns the choice with timings."""
+        """Pick the ensemble launch shapes (elements per X-drift and kick work
+        item, persistent workgroups per CU) by timing each candidate on a copy
+        of the pool; sets the library knobs and returns the choice with
+        timings."""
         L = lib()
         K = _lib.KNOBS
         scratch = self.pool.clone()
@@ -151,25 +152,20 @@ class Ensemble:
 
         results = {}
         best = None
-        for cols in (16, 32):
-            for xt in (256, 512, 1024):
-                L.vp_tune_set(K["ens_cols"], cols)
-                L.vp_tune_set(K["ens_xthreads"], xt)
-                t = timed()
-                results[f"cols={cols},xthreads={xt}"] = t
-                if best is None or t < best[0]:
-                    best = (t, cols, xt)
-        L.vp_tune_set(K["ens_cols"], best[1])
-        L.vp_tune_set(K["ens_xthreads"], best[2])
-        best_r = None
-        for rows in (2, 4, 8, 16):
-            L.vp_tune_set(K["ens_rows"], rows)
-            t = timed()
-            results[f"rows={rows}"] = t
-            if best_r is None or t < best_r[0]:
-                best_r = (t, rows)
-        L.vp_tune_set(K["ens_rows"], best_r[1])
+        for xt in (2048, 4096):
+            for kt in (2048, 4096):
+                for w in (1, 2):
+                    L.vp_tune_set(K["ens_xtile"], xt)
+                    L.vp_tune_set(K["ens_ktile"], kt)
+                    L.vp_tune_set(K["ens_wgs"], w)
+                    t = timed()
+                    results[f"xtile={xt},ktile={kt},wgs={w}"] = t
+                    if best is None or t < best[0]:
+                        best = (t, xt, kt, w)
+        L.vp_tune_set(K["ens_xtile"], best[1])
+        L.vp_tune_set(K["ens_ktile"], best[2])
+        L.vp_tune_set(K["ens_wgs"], best[3])
         self._graphs.clear()  # captured graphs hold the old launch shapes
-        # the workspace bound covers both chunk widths, so it stays valid
-        return {"ens_cols": best[1], "ens_xthreads": best[2], "ens_rows": best_r[1], "step_s": best_r[0],
+        # the workspace bound covers every tile size, so it stays valid
+        return {"ens_xtile": best[1], "ens_ktile": best[2], "ens_wgs": best[3], "step_s": best[0],
                 "candidates_s": results}

@@ -18,8 +18,8 @@ from oracle import vlasov_oracle as vo
 
 pytestmark = pytest.mark.gpu
 
-# ragged sizes: not multiples of the 16/32-column chunks or the kick row blocks,
-# the extremes of the supported range, nv below one chunk
+# ragged sizes: not multiples of the X items' column blocks or the kick's row
+# blocks, the extremes of the supported range, nv below one column block
 SIZES = [(32, 64), (37, 70), (100, 129), (64, 512), (256, 100), (48, 33), (6, 6), (129, 257)]
 
 
@@ -104,23 +104,24 @@ def test_ensemble_graph_replays_match_eager(device):
 def test_autotuned_shapes_keep_results(device):
     from vlasov_amd import _lib
     lib = _lib.lib()
-    keys = ("ens_cols", "ens_rows", "ens_xthreads")
+    keys = ("ens_xtile", "ens_ktile", "ens_wgs")
     old = {k: lib.vp_tune_get(_lib.KNOBS[k]) for k in keys}
     try:
         a = _ens()
         choice = a.autotune(steps=1, reps=1)
-        assert choice["ens_cols"] in (16, 32) and choice["ens_rows"] in (2, 4, 8, 16)
+        assert choice["ens_xtile"] in (2048, 4096) and choice["ens_ktile"] in (2048, 4096)
+        assert choice["ens_wgs"] in (1, 2)
         a.run(6)
         for k in keys:
             lib.vp_tune_set(_lib.KNOBS[k], 0)
         b = _ens()
         b.run(6)
-        # the chunk width changes the charge-density summation order only
+        # the X item width changes the charge-density summation order only
         assert float((a.pool - b.pool).abs().max() / b.pool.abs().max()) <= 1e-13
-        for cols, rows, xt in ((16, 2, 256), (32, 16, 1024)):
-            lib.vp_tune_set(_lib.KNOBS["ens_cols"], cols)
-            lib.vp_tune_set(_lib.KNOBS["ens_rows"], rows)
-            lib.vp_tune_set(_lib.KNOBS["ens_xthreads"], xt)
+        for xt, kt, w in ((2048, 4096, 2), (4096, 2048, 1)):
+            lib.vp_tune_set(_lib.KNOBS["ens_xtile"], xt)
+            lib.vp_tune_set(_lib.KNOBS["ens_ktile"], kt)
+            lib.vp_tune_set(_lib.KNOBS["ens_wgs"], w)
             c = _ens()
             c.run(6)
             assert float((c.pool - b.pool).abs().max() / b.pool.abs().max()) <= 1e-13
