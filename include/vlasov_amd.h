@@ -439,9 +439,9 @@ enum {
                                   than 256: 256..4096 indices per workgroup
                                   segment (0 = 2048 for lines of 2048 or
                                   more, else 1024)                             */
-    VP_KNOB_F32_PAIRS = 11,    /* float32 X drift: pairs of vy columns moved
+    VP_KNOB_F32_PAIRS = 11,    /* float32 drifts: pairs of vy columns moved
                                   as 8-byte elements, 1 = never, 2 = always
-                                  (0 = for the two-pass drift only)            */
+                                  (0 = the Y drifts and the two-pass X drift)  */
     VP_KNOB_POISSON_DFT = 12,  /* 1: 2D Poisson by the exact-twiddle DFT kernels
                                   also for power-of-two grids (default: radix-2
                                   FFTs there); results agree to rounding      */

@@ -267,6 +267,43 @@ __device__ __forceinline__ E ring_tap_sum(const E* tile, uint32_t A, const S (&w
     return tap_sum_e(w, v);
 }
 
+// A pair of float32 columns with weights of their own (w0, w1), taps from
+// the same rows (their departure rows agree): the two scalar sums.
+__device__ __forceinline__ f32x2 tap_sum_e2(const float (&w0)[6], const float (&w1)[6], const f32x2 (&v)[6]) {
+    float a[6], b[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        a[k] = v[k].x;
+        b[k] = v[k].y;
+    }
+    f32x2 o;
+    o.x = tap_sum<float>(w0, a);
+    o.y = tap_sum<float>(w1, b);
+    return o;
+}
+
+template <bool DIRECT, int N>
+__device__ __forceinline__ f32x2 ring_pair_sum(const f32x2* tile, uint32_t A, const float (&w0)[6],
+                                               const float (&w1)[6]) {
+    const char* base = reinterpret_cast<const char*>(tile);
+    f32x2 v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        v[k] = *reinterpret_cast<const f32x2*>(base + (DIRECT ? A + 256u * k : (A + 256u * k) & (256u * N - 1)));
+    return tap_sum_e2(w0, w1, v);
+}
+
+// One float32 column's six taps at byte offset A (tap-0 row) of the ring
+// tile, wrapping: for a pair whose two columns depart from different rows.
+template <int N>
+__device__ __forceinline__ float ring_col_sum(const f32x2* tile, uint32_t A, const float (&w)[6]) {
+    const char* base = reinterpret_cast<const char*>(tile);
+    float v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = *reinterpret_cast<const float*>(base + ((A + 256u * k) & (256u * N - 1)));
+    return tap_sum<float>(w, v);
+}
+
 // Same sweep as drift_kernel, but the tiles stream into an LDS ring of T
 // slots with direct-to-LDS loads (global_load_lds_dwordx4: each lane moves 16
 // bytes, a wave fills 1 KB = 4 tile rows), so T - 1 tiles are in flight per
@@ -286,16 +323,23 @@ __device__ __forceinline__ E ring_tap_sum(const E* tile, uint32_t A, const S (&w
 // so the count stays the same to the end; for T = 2 the only load a wait
 // depends on is the one issued in the iteration before, and the last
 // iteration issues none.
-// VW = 2 (float32 storage, weights shared along the tile row: the X drift):
-// each thread handles a pair of neighbouring columns as one 8-byte element,
-// with the float64 drift's thread layout; results are those of VW = 1.
+// VW = 2 (float32 storage): each thread handles a pair of neighbouring
+// columns as one 8-byte element, with the float64 drift's thread layout;
+// results are those of VW = 1.  X drift: the pair shares its weights (they
+// depend on the row's velocity).  Y drift (LANE_W): each column of the pair
+// has its own weights and departure row; where the two departure rows differ
+// for some lane of a wave (columns either side of a change of floor(-disp)),
+// that wave reads the row's taps column by column.
 template <typename S, int R, int PASSES, bool RHO, bool LANE_W, int T, bool NT, int VW = 1>
 __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     const S* fin_s, S* fout_s, int nq, int nsplit, int64_t a_stride, int64_t row_stride,
     int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
     double2* __restrict__ partial) {
     using E = typename LaneElem<S, VW>::type;
-    static_assert(VW == 1 || (!RHO && !LANE_W), "column pairs share one weight set");
+    constexpr bool PAIRW = VW == 2 && LANE_W;  // pair with a weight set per column
+    static_assert(VW == 1 || !RHO || PAIRW, "column pairs: rho with the Y drift only");
+    static_assert(!(VW == 2 && LANE_W) || sizeof(S) == 4, "Y-drift column pairs: float32");
+    static_assert(!PAIRW || PASSES == 1, "Y drift: one pass");
     const E* fin = reinterpret_cast<const E*>(fin_s);
     E* fout = reinterpret_cast<E*>(fout_s);
     a_stride /= VW;  // strides in elements of E
@@ -321,10 +365,13 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     const int q = b % nq;
     const int a = b / nq;
     const int64_t base = static_cast<int64_t>(a) * a_stride + q * KC + c;
-    const int j = LANE_W ? q * KC + c : a;
+    const int j = LANE_W ? (PAIRW ? 2 * (q * KC + c) : q * KC + c) : a;
 
-    S w[R][6];  // weights (shared by the pair when VW = 2)
+    S w[R][6];  // weights (of the first column when PAIRW, else shared by a pair)
     uint32_t A[R];
+    float w1[PAIRW ? R : 1][6];   // PAIRW: the second column's weights,
+    uint32_t A1[PAIRW ? R : 1];   // the byte offset of its tap-0 row,
+    bool split[PAIRW ? R : 1];    // and whether that row differs from the first column's
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         const int r = g + KG * i;
@@ -334,6 +381,14 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
         round_weights<S>(wd, w[i]);
         const int il = IL[static_cast<int64_t>(r) * nvel + j];
         A[i] = static_cast<uint32_t>(((il - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(E)));
+        if constexpr (PAIRW) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wd[k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j + 1];
+            round_weights<float>(wd, w1[i]);
+            const int il1 = IL[static_cast<int64_t>(r) * nvel + j + 1];
+            A1[i] = static_cast<uint32_t>(((il1 - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(E)) + 4);
+            split[i] = il1 != il;
+        }
     }
     // Consume the weight loads here: otherwise the wait-count analysis of the
     // loop body also waits for the in-flight stores before each FMA.
@@ -342,7 +397,15 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
 #pragma unroll
         for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w[i][k]));
         asm volatile("" ::"v"(A[i]));
+        if constexpr (PAIRW) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w1[i][k]));
+            asm volatile("" ::"v"(A1[i]));
+        }
     }
+    bool wsplit[PAIRW ? R : 1];  // some lane of the wave reads the row column by column
+#pragma unroll
+    for (int i = 0; i < (PAIRW ? R : 1); ++i) wsplit[i] = PAIRW && __any(split[i]);
     // float32 storage: rows whose taps wrap for no lane of the wave (all but
     // a few next to the line ends) read them at immediate offsets, dropping
     // two integer operations per tap; float32 interpolation costs as many
@@ -412,14 +475,29 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
         }
 #pragma unroll
         for (int ii = 0; ii < R; ++ii) {
-            const E o = direct[ii] ? ring_tap_sum_direct<E, S>(tile, A[ii], w[ii])
-                                   : ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
+            E o;
+            if constexpr (PAIRW) {
+                if (wsplit[ii]) {
+                    o.x = ring_col_sum<n>(tile, A[ii], w[ii]);
+                    o.y = ring_col_sum<n>(tile, A1[ii], w1[ii]);
+                } else if (direct[ii]) {
+                    o = ring_pair_sum<true, n>(tile, A[ii], w[ii], w1[ii]);
+                } else {
+                    o = ring_pair_sum<false, n>(tile, A[ii], w[ii], w1[ii]);
+                }
+            } else {
+                o = direct[ii] ? ring_tap_sum_direct<E, S>(tile, A[ii], w[ii]) : ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
+            }
             store_f<NT>(&fout[base + (g + KG * ii) * row_stride + l * loop_stride], o);
             if constexpr (RHO) {
-                if (COMP)
+                if constexpr (PAIRW) {
+                    acc[ii].s += static_cast<double>(o.x);
+                    acc[ii].s += static_cast<double>(o.y);
+                } else if (COMP) {
                     ksum_add(acc[ii], static_cast<double>(o));
-                else
+                } else {
                     acc[ii].s += static_cast<double>(o);
+                }
             }
         }
     }
@@ -1178,11 +1256,22 @@ static int ydrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs&
     const int64_t a_stride = ny * nvx * nvy, row_stride = nvx * nvy, loop_stride = nvy;
     const int nvel = static_cast<int>(nvy), lt = static_cast<int>(nvx);
     int rc = VP_ERR_UNSUPPORTED;
+    // float32 storage: a thread takes a pair of vy columns (8-byte loads and
+    // stores, a weight set per column) unless the pairs knob is 1
+    const bool pairs = sizeof(S) == 4 && knob(VP_KNOB_F32_PAIRS) != 1 &&
+                       ((reinterpret_cast<uintptr_t>(fin) | reinterpret_cast<uintptr_t>(fout)) & 7) == 0;
 #define VP_Y(N, RH)                                                                                              \
     if (ny == N && (N) % KG == 0) {                                                                              \
         constexpr int RV = (N) / KG > 0 ? (N) / KG : 1;                                                          \
-        rc = launch_drift<S, RV, 1, RH, true>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride, lt,   \
-                                              L.Wy, L.ILy, nvel, L.partial, s);                                  \
+        constexpr int RV2 = (N) / (2 * KG) > 0 ? (N) / (2 * KG) : 1;                                             \
+        if constexpr (sizeof(S) == 4 && (N) >= 64) {                                                             \
+            if (pairs)                                                                                           \
+                rc = launch_drift<S, RV2, 1, RH, true, 2>(fin, fout, blocks, nq, S_, a_stride, row_stride,       \
+                                                          loop_stride, lt, L.Wy, L.ILy, nvel, L.partial, s);     \
+        }                                                                                                        \
+        if (rc == VP_ERR_UNSUPPORTED)                                                                            \
+            rc = launch_drift<S, RV, 1, RH, true>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride,  \
+                                                  lt, L.Wy, L.ILy, nvel, L.partial, s);                          \
     }
     if (rho) {
         VP_Y(128, true) else VP_Y(64, true) else VP_Y(32, true)

@@ -12,6 +12,12 @@ simulations/vlasov1d_1v/postscript/postscript.py:104-180), inferring the
 element type from the file size.  Files written here follow it exactly, so
 those scripts read them unchanged; ``read_snapshot`` restores a state from
 them (or from the Kokkos solver's own dumps).
+
+The density files hold the Kokkos solver's quantity, the full density
+rho = sum_v f dv (``compute_rho``, simulations/vlasov1d_1v/kokkos/src/
+vlasov_poisson.hpp:418), not the PyTorch drivers' delta_rho = rho - 1:
+the drivers write ``kokkos_rho(delta_rho)`` and convert back on restart with
+``delta_rho_of(rho)``.
 """
 from __future__ import annotations
 
@@ -23,13 +29,23 @@ import torch
 VARIABLES = ("fn", "f0", "rho", "phi")
 
 
+def kokkos_rho(delta_rho):
+    """Full density rho = delta_rho + 1, the Kokkos snapshots' rho."""
+    return delta_rho + 1.0
+
+
+def delta_rho_of(rho):
+    """delta_rho = rho - 1 of a Kokkos-convention density."""
+    return rho - 1.0
+
+
 def snapshot_path(out_dir, name: str, step: int, suffix: str = "bin") -> pathlib.Path:
     return pathlib.Path(out_dir) / f"{name}_{int(step):06d}.{suffix}"
 
 
 def write_snapshot(out_dir, step: int, fn, f0, rho, phi) -> list:
     """Write the four arrays (tensors or arrays) as raw C-order files; returns
-    the paths written."""
+    the paths written.  rho is the full density (``kokkos_rho``)."""
     out = pathlib.Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     paths = []

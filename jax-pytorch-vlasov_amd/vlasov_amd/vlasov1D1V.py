@@ -537,7 +537,7 @@ class Vlasov:
         if self.snapshot_format == "bin":
             from . import snapshots
             snapshots.write_snapshot(self.out_dir, state.step, state.variables.fn, state.variables.f0,
-                                     state.field.rho, state.field.phi)
+                                     snapshots.kokkos_rho(state.field.rho), state.field.phi)
             return
         filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
         np.savez(filename, f=state.variables.fn.detach().cpu().numpy(),
@@ -555,6 +555,7 @@ class Vlasov:
         a = snapshots.read_snapshot(self.out_dir, step, {"fn": (g.Nx, g.Nv), "f0": (g.Nv,), "rho": (g.Nx,),
                                                          "phi": (g.Nx,)})
         t = {k: torch.from_numpy(v.astype(np.float64)).to(self.device) for k, v in a.items()}
+        t["rho"] = snapshots.delta_rho_of(t["rho"])  # the files hold the full density
         return SimulationState(field=Field(t["rho"], t["phi"]), variables=Variables(fn=t["fn"], f0=t["f0"]),
                                t=float(step) * self.dt, step=int(step))
 

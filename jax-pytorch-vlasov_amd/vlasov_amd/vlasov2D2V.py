@@ -521,7 +521,7 @@ class Vlasov:
         if self.snapshot_format == "bin":
             from . import snapshots
             snapshots.write_snapshot(self.out_dir, state.step, state.variables.fn, state.variables.f0,
-                                     state.field.rho, state.field.phi)
+                                     snapshots.kokkos_rho(state.field.rho), state.field.phi)
             return
         filename = pathlib.Path(self.out_dir) / f"field_{int(state.step):06d}.npz"
         np.savez(filename, f=state.variables.fn.detach().cpu().numpy(), f0=state.variables.f0.detach().cpu().numpy(),
@@ -542,6 +542,7 @@ class Vlasov:
                                                          "rho": (g.Nx, g.Ny), "phi": (g.Nx, g.Ny)})
         fn = torch.from_numpy(a["fn"]).to(self.device).to(self.f_dtype)
         t = {k: torch.from_numpy(a[k].astype(np.float64)).to(self.device) for k in ("f0", "rho", "phi")}
+        t["rho"] = snapshots.delta_rho_of(t["rho"])  # the files hold the full density
         return SimulationState(field=Field(t["rho"], t["phi"]), variables=Variables(fn=fn, f0=t["f0"]),
                                t=float(step) * self.dt, step=int(step))
 
