@@ -139,12 +139,14 @@ struct KSum {
     double c;
 };
 
+// Neumaier's compensated sum: c collects the exact rounding error of every
+// s + x.  The error is computed by Knuth's branch-free TwoSum, which yields
+// the same exact value as Neumaier's |s| >= |x| branches (both are exact),
+// without the comparison and selects.
 __device__ __forceinline__ void ksum_add(KSum& k, double x) {
     const double t = k.s + x;
-    if (fabs(k.s) >= fabs(x))
-        k.c += (k.s - t) + x;
-    else
-        k.c += (x - t) + k.s;
+    const double z = t - k.s;
+    k.c += (k.s - (t - z)) + (x - z);
     k.s = t;
 }
 
