@@ -445,7 +445,13 @@ enum {
     VP_KNOB_POISSON_DFT = 12,  /* 1: 2D Poisson by the exact-twiddle DFT kernels
                                   also for power-of-two grids (default: radix-2
                                   FFTs there); results agree to rounding      */
-    VP_KNOB_COUNT = 13
+    VP_KNOB_Y_UNITS = 13,      /* Y drift over 128-row lines: 1 = one workgroup
+                                  per (x, vx range); 2 = persistent workgroups
+                                  (one per CU) walking ranges of them with
+                                  rotated starts; 3 = 2 with the vy chunks
+                                  spread over the XCDs; 4 = 3 unrotated
+                                  (0: 2 for float32 storage, else 1)          */
+    VP_KNOB_COUNT = 14
 };
 int vp_tune_set(int knob, int value);
 int vp_tune_get(int knob);

@@ -330,9 +330,23 @@ __device__ __forceinline__ float ring_col_sum(const f32x2* tile, uint32_t A, con
 // has its own weights and departure row; where the two departure rows differ
 // for some lane of a wave (columns either side of a change of floor(-disp)),
 // that wave reads the row's taps column by column.
+// Units: the loop range of outer index a splits into nsplit parts; unit
+// u = a * nsplit + s covers tiles [s * loop_total / nsplit, (s + 1) * ...).
+// With one workgroup per unit (gridDim.x = na * nq * nsplit) workgroup b
+// takes s = b % nsplit, q = (b / nsplit) % nq, a = b / (nsplit * nq).  With
+// fewer (persistent) workgroups, G per column chunk (gridDim.x = nq * G; the
+// Y drift only: its weights depend on the column, not on a), workgroup b
+// takes chunk q = b % nq and walks the units [g U / G, (g + 1) U / G),
+// g = b / nq, U = na * nsplit, as one stream of tiles: the ring prefetches
+// across unit boundaries, the weights stay in registers, and ROT rotates the
+// start of each unit's loop range by (13 u) mod its length, so that the
+// workgroups running at one time read different loop indices (all starting
+// at 0, they would read the same offsets within their rows at once).
+// RHO: at the end of a unit the row sums of its tiles over the chunk's
+// columns go to partial[(q * nsplit + s) * na * n + a * n + row].
 template <typename S, int R, int PASSES, bool RHO, bool LANE_W, int T, bool NT, int VW = 1>
 __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
-    const S* fin_s, S* fout_s, int nq, int nsplit, int64_t a_stride, int64_t row_stride,
+    const S* fin_s, S* fout_s, int nq, int nsplit, int na, int rot, int64_t a_stride, int64_t row_stride,
     int64_t loop_stride, int loop_total, const double* __restrict__ W, const int* __restrict__ IL, int nvel,
     double2* __restrict__ partial) {
     using E = typename LaneElem<S, VW>::type;
@@ -359,13 +373,31 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     const int g = t / KC;
     const int wave = t >> 6;
     const int lane = t & 63;
-    int b = blockIdx.x;
-    const int s = b % nsplit;
-    b /= nsplit;
-    const int q = b % nq;
-    const int a = b / nq;
-    const int64_t base = static_cast<int64_t>(a) * a_stride + q * KC + c;
-    const int j = LANE_W ? (PAIRW ? 2 * (q * KC + c) : q * KC + c) : a;
+    const int U = na * nsplit;
+    int q, u_begin, u_end;
+    if (static_cast<int>(gridDim.x) == U * nq) {  // one workgroup per unit
+        const int b = blockIdx.x;
+        q = (b / nsplit) % nq;
+        u_begin = (b / (nsplit * nq)) * nsplit + b % nsplit;
+        u_end = u_begin + 1;
+    } else {
+        const int G = static_cast<int>(gridDim.x) / nq, b = static_cast<int>(blockIdx.x);
+        int gw;
+        if ((rot & 2) && G % 8 == 0) {
+            // consecutive workgroups (dealt round-robin to the 8 XCDs) in
+            // groups of 8 per chunk: every XCD works on every chunk
+            q = (b / 8) % nq;
+            gw = (b / (8 * nq)) * 8 + b % 8;
+        } else {
+            q = b % nq;
+            gw = b / nq;
+        }
+        u_begin = static_cast<int>(static_cast<int64_t>(gw) * U / G);
+        u_end = static_cast<int>(static_cast<int64_t>(gw + 1) * U / G);
+    }
+    if (u_begin >= u_end) return;  // uniform over the workgroup
+    const int64_t base = q * KC + c;  // + a * a_stride of the unit
+    const int j = LANE_W ? (PAIRW ? 2 * (q * KC + c) : q * KC + c) : u_begin / nsplit;
 
     S w[R][6];  // weights (of the first column when PAIRW, else shared by a pair)
     uint32_t A[R];
@@ -415,9 +447,31 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     bool direct[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) direct[i] = DIRECT && __all((A[i] >> 8) <= static_cast<uint32_t>(n - 6));
-    const int l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / nsplit);
-    const int l1 = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / nsplit);
-    if (l0 >= l1) return;  // uniform over the workgroup
+
+    // Walk over the units' tiles (wave-uniform): tile k of unit u is loop
+    // index l0 + (r0 + k) mod len.
+    struct Walk {
+        int u, l0, len, r0, k;
+    };
+    auto start = [&](Walk& w, int u) {
+        const int s = u % nsplit;
+        w.u = u;
+        w.l0 = static_cast<int>(static_cast<int64_t>(s) * loop_total / nsplit);
+        w.len = static_cast<int>(static_cast<int64_t>(s + 1) * loop_total / nsplit) - w.l0;
+        w.r0 = (rot & 1) && w.len > 0 ? (13 * u) % w.len : 0;
+        w.k = 0;
+    };
+    auto tile_of = [](const Walk& w) {
+        const int x = w.r0 + w.k;
+        return w.l0 + (x >= w.len ? x - w.len : x);
+    };
+    int ntiles = 0;
+    for (int u = u_begin; u < u_end; ++u) {
+        Walk w;
+        start(w, u);
+        ntiles += w.len;
+    }
+    if (ntiles == 0) return;  // uniform over the workgroup
 
     // Load instruction k of this wave covers tile rows 4m .. 4m+3, m = wave + 16k;
     // lane -> row 4m + lane/16, the 16 bytes at column PER_LANE*(lane%16).
@@ -425,18 +479,26 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const int row = 4 * (wave + 16 * k) + (lane >> 4);
-        src_lane[k] = fin + static_cast<int64_t>(a) * a_stride + q * KC + PER_LANE * (lane & 15) + row * row_stride;
+        src_lane[k] = fin + q * KC + PER_LANE * (lane & 15) + row * row_stride;
     }
     uint32_t dst_wave[T];
 #pragma unroll
     for (int u = 0; u < T; ++u) dst_wave[u] = __builtin_amdgcn_readfirstlane(lds_addr(&ring[u][4 * wave * KC]));
-    auto issue = [&](int tile_l, int slot) {
+    // Loads the tile under the issue walk into `slot` and moves the walk on;
+    // at the end of the last unit it stays on the last tile.
+    Walk iw;
+    start(iw, u_begin);
+    auto issue = [&](int slot) {
+        const int64_t off = static_cast<int64_t>(iw.u / nsplit) * a_stride + static_cast<int64_t>(tile_of(iw)) * loop_stride;
 #pragma unroll
-        for (int k = 0; k < P; ++k)
-            load_lds_x4<NT>(src_lane[k] + static_cast<int64_t>(tile_l) * loop_stride, dst_wave[slot] + k * (16 * 1024));
+        for (int k = 0; k < P; ++k) load_lds_x4<NT>(src_lane[k] + off, dst_wave[slot] + k * (16 * 1024));
+        if (iw.k + 1 < iw.len)
+            ++iw.k;
+        else if (iw.u + 1 < u_end)
+            start(iw, iw.u + 1);
     };
 #pragma unroll
-    for (int u = 0; u < T - 1; ++u) issue(l0 + u < l1 ? l0 + u : l1 - 1, u);
+    for (int u = 0; u < T - 1; ++u) issue(u);
 
     // Charge density of this thread's rows: compensated for float64 storage;
     // for float32 storage the float64 sum of at most 128 float32 values per
@@ -445,8 +507,33 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     KSum acc[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) acc[i] = KSum{0.0, 0.0};
-    for (int l = l0; l < l1; ++l) {
-        const int i = l - l0;
+    // End of a unit: the row sums of its tiles over the chunk's KC columns
+    // (the lanes of one row group: a butterfly of lane shuffles), written by
+    // the group's column-0 lane.  Its stores join the vmcnt queue behind the
+    // tile stores; the waits below then wait for a little more than they
+    // need to, never less.
+    auto flush = [&](int u) {
+        const int64_t at = static_cast<int64_t>(q * nsplit + u % nsplit) * na * n + static_cast<int64_t>(u / nsplit) * n;
+#pragma unroll
+        for (int ii = 0; ii < R; ++ii) {
+            KSum k = acc[ii];
+            for (int off = KC / 2; off > 0; off >>= 1) {
+                KSum o;
+                o.s = __shfl_xor(k.s, off, KC);
+                if (COMP) {
+                    o.c = __shfl_xor(k.c, off, KC);
+                    ksum_merge(k, o);
+                } else {
+                    k.s += o.s;
+                }
+            }
+            if (c == 0) partial[at + g + KG * ii] = make_double2(k.s, COMP ? k.c : 0.0);
+            acc[ii] = KSum{0.0, 0.0};
+        }
+    };
+    Walk ow;  // the tile computed
+    start(ow, u_begin);
+    for (int i = 0; i < ntiles; ++i) {
         const int slot = i % T;
         if (i == 0) wait_vmcnt<(T - 2) * P>();
         else if (T > 2 && i == 1) wait_vmcnt<(T - 2) * P + R>();
@@ -456,11 +543,12 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
         if (T == 2) {
             // with two slots nothing waits for a load issued in the last
             // iteration, so the tail issues none
-            if (l + 1 < l1) issue(l + 1, (i + 1) % T);
+            if (i + 1 < ntiles) issue((i + 1) % T);
         } else {
-            const int ln = l + T - 1 < l1 ? l + T - 1 : l1 - 1;
-            issue(ln, (i + T - 1) % T);
+            issue((i + T - 1) % T);  // past the end: the last tile again
         }
+        const int l = tile_of(ow);
+        const int64_t obase = base + static_cast<int64_t>(ow.u / nsplit) * a_stride;
         const E* tile = ring[slot];
 #pragma unroll
         for (int ii = 0; ii < R; ++ii) asm volatile("" : "+v"(A[ii]));  // keep the tap offsets in the loop
@@ -488,7 +576,7 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
             } else {
                 o = direct[ii] ? ring_tap_sum_direct<E, S>(tile, A[ii], w[ii]) : ring_tap_sum<E, S, n>(tile, A[ii], w[ii]);
             }
-            store_f<NT>(&fout[base + (g + KG * ii) * row_stride + l * loop_stride], o);
+            store_f<NT>(&fout[obase + (g + KG * ii) * row_stride + l * loop_stride], o);
             if constexpr (RHO) {
                 if constexpr (PAIRW) {
                     acc[ii].s += static_cast<double>(o.x);
@@ -500,57 +588,14 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
                 }
             }
         }
+        if (ow.k + 1 < ow.len) {
+            ++ow.k;
+        } else {
+            if constexpr (RHO) flush(ow.u);
+            if (ow.u + 1 < u_end) start(ow, ow.u + 1);
+        }
     }
     wait_vmcnt<0>();  // no direct-to-LDS load may still be in flight when the workgroup ends
-    if constexpr (RHO) {
-        // Row sums over the KC columns through LDS (the ring is free now):
-        // every thread parks its R partial sums at [row][column], then
-        // 1024 / n threads per row add KC * n / 1024 columns each and merge
-        // with a few lane shuffles.  Float64 storage keeps (sum, compensation)
-        // pairs; float32 storage plain float64 sums.
-        constexpr int TPR = kDriftThreads / n;  // threads per row
-        constexpr int CPT = KC / TPR;           // columns per thread
-        static_assert(CPT * TPR == KC && TPR <= 64, "row reduction layout");
-        constexpr int W2 = COMP ? 2 : 1;  // doubles per parked sum
-        static_assert(sizeof(ring) >= sizeof(double) * W2 * n * KC, "ring too small for the row sums");
-        double* red = reinterpret_cast<double*>(&ring[0][0]);
-        __syncthreads();  // every wave is done with the ring
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            double* e = red + ((g + KG * i) * KC + c) * W2;
-            e[0] = acc[i].s;
-            if (COMP) e[1] = acc[i].c;
-        }
-        __syncthreads();
-        const int row = t / TPR;
-        const int c0 = (t % TPR) * CPT;
-        KSum k{0.0, 0.0};
-#pragma unroll
-        for (int u = 0; u < CPT; ++u) {
-            const double* e = red + (row * KC + c0 + u) * W2;
-            if (COMP)
-                ksum_merge(k, KSum{e[0], e[1]});
-            else
-                k.s += e[0];
-        }
-        for (int off = TPR / 2; off > 0; off >>= 1) {
-            KSum o;
-            o.s = __shfl_xor(k.s, off, TPR);
-            if (COMP) {
-                o.c = __shfl_xor(k.c, off, TPR);
-                ksum_merge(k, o);
-            } else {
-                k.s += o.s;
-            }
-        }
-        if (t % TPR == 0) {
-            const int nparts = nq * nsplit;
-            // part-major layout [part][x * ny + y]: the Poisson kernel reads it coalesced
-            const int64_t npoints = static_cast<int64_t>(gridDim.x / nparts) * n;
-            partial[static_cast<int64_t>(q * nsplit + s) * npoints + static_cast<int64_t>(a) * n + row] =
-                make_double2(k.s, COMP ? k.c : 0.0);
-        }
-    }
 }
 
 // Dirichlet tap (clamp + zero mask, vlasov2D2V.py:340-345).
@@ -1065,7 +1110,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
 // Host side
 // ---------------------------------------------------------------------------
 // Launch-shape knobs (vp_tune_set); 0 = automatic choice.
-static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+static int g_knobs[VP_KNOB_COUNT] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 int knob(int k) { return (k >= 0 && k < VP_KNOB_COUNT) ? g_knobs[k] : 0; }
 
@@ -1108,22 +1153,54 @@ bool fast2d_supported_f32(const int64_t shape[4]) {
 
 static inline size_t a256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
-// Number of vx ranges the Y drift splits its loop into.
-static int y_split(int64_t nvx, int elem_bytes) {
+// Launch plan of the Y drift: nsplit = vx ranges per x (the units; it fixes
+// the rho partial layout), groups = workgroups per vy chunk, rot = walk
+// options of persistent workgroups (bit 0: rotated unit starts, bit 1:
+// chunks spread over the XCDs).  Measured at 128^4 (alternating runs):
+// * float32 storage: persistent workgroups, one per CU (the register file
+//   holds one), two units each (2 vx ranges per x), rotated starts: Y drift
+//   + rho 0.49-0.51 -> 0.42 ms, Y drift 0.43-0.44 -> 0.43 ms, step 1.85 ->
+//   1.71 ms; one unit each was slower (0.53 ms Y drift), so were 4 ranges,
+//   the XCD spread and unrotated starts (workgroups that start together then
+//   read the same offsets of their rows at once).
+// * float64: one workgroup per unit, 12 ranges per x (10-11 tiles each; 1, 8,
+//   16, 32 slower): persistent workgroups took 0.89 ms per Y drift + rho
+//   against 0.83-0.84 (any range count, rotation or spread).
+// * 64^4: one workgroup per unit, 12 ranges for both storage types (2-8,
+//   16, 24, 32 slower; float32 Y drifts 9 % faster than at 5).
+// Knob VP_KNOB_Y_UNITS: 1 = one workgroup per unit, 2 = persistent, 3 =
+// persistent and spread, 4 = persistent, spread, unrotated (0 = as above).
+struct YPlan {
+    int nsplit;
+    int groups;
+    int rot;
+};
+
+static YPlan y_plan(int64_t nx, int64_t ny, int64_t nvx, int64_t nvy, int elem_bytes) {
+    const int nq = static_cast<int>(nvy / (256 / elem_bytes));
+    const int mode = knob(VP_KNOB_Y_UNITS);
+    const bool persistent = ny == 128 && nq >= 1 && (mode >= 2 || (mode == 0 && elem_bytes == 4));
     int S = knob(VP_KNOB_Y_SPLIT);
     if (S <= 0) {
-        // Measured: at 128^4 float64 best at 12 splits (10-11 tiles per
-        // workgroup; 1, 8, 16, 32 slower), float32 storage at 10 (6, 8, 12, 16
-        // slower); at 64^4 both at 12 (2-8, 16, 24, 32 slower; float32 Y
-        // drifts 9 % faster than at 5).
-        if (nvx == 64)
+        if (persistent) {
+            const int gq = cu_count() / nq;  // two units per workgroup
+            S = 2 * gq > nx ? static_cast<int>(2 * gq / nx) : 1;
+        } else if (nvx == 64) {
             S = 12;
-        else
+        } else {
             S = elem_bytes == 8 ? static_cast<int>((3 * nvx + 16) / 32) : static_cast<int>((5 * nvx + 32) / 64);
+        }
         if (S < 1) S = 1;
     }
     if (S > nvx) S = static_cast<int>(nvx);
-    return S;
+    const int units = static_cast<int>(nx) * S;
+    YPlan p{S, units, 0};
+    if (persistent) {
+        const int gq = cu_count() / (nq > 0 ? nq : 1);
+        p.groups = gq >= 1 && gq < units ? gq : units;
+        p.rot = mode == 3 ? 3 : (mode == 4 ? 2 : 1);
+    }
+    return p;
 }
 
 Fast2DWs fast2d_layout(const int64_t shape[4], void* ws, int elem_bytes, bool with_ftmp) {
@@ -1141,7 +1218,7 @@ Fast2DWs fast2d_layout(const int64_t shape[4], void* ws, int elem_bytes, bool wi
     L.ILx = reinterpret_cast<int*>(take(static_cast<size_t>(nx) * nvx * sizeof(int)));
     L.Wy = reinterpret_cast<double*>(take(static_cast<size_t>(ny) * 6 * nvy * sizeof(double)));
     L.ILy = reinterpret_cast<int*>(take(static_cast<size_t>(ny) * nvy * sizeof(int)));
-    L.nparts = static_cast<int>((nvy / col) * y_split(nvx, elem_bytes));
+    L.nparts = static_cast<int>((nvy / col) * y_plan(nx, ny, nvx, nvy, elem_bytes).nsplit);
     L.partial = reinterpret_cast<double2*>(take(static_cast<size_t>(nx) * ny * L.nparts * sizeof(double2)));
     L.pws_bytes = poisson2d_ws(nx, ny);
     L.pws = take(L.pws_bytes);
@@ -1168,9 +1245,12 @@ int fast2d_tables(const int64_t shape[4], const Fast2DWs& L, const double* vx, c
 // fin is not 16-byte aligned (float64 only); otherwise the register-prefetch
 // kernel (float64 only).  R = rows per thread.
 template <typename S, int R, int PASSES, bool RHO, bool LANE_W, int VW = 1>
-static int launch_drift(const S* fin, S* fout, int64_t blocks, int nq, int nsplit, int64_t a_stride,
+static int launch_drift(const S* fin, S* fout, int nq, int nsplit, int na, int groups, int rot, int64_t a_stride,
                         int64_t row_stride, int64_t loop_stride, int lt, const double* W, const int* IL, int nvel,
                         double2* partial, hipStream_t s) {
+    const int64_t unit_blocks = static_cast<int64_t>(na) * nq * nsplit;  // one workgroup per unit
+    if (!LANE_W || groups > na * nsplit) groups = na * nsplit;  // X drift: weights depend on a
+    const int64_t blocks = static_cast<int64_t>(nq) * groups;
     constexpr bool f64 = sizeof(S) == 8;
     constexpr int rows = R * RingGeom<typename LaneElem<S, VW>::type>::groups;
     const int mode = knob(VP_KNOB_DRIFT_DMA);
@@ -1180,7 +1260,8 @@ static int launch_drift(const S* fin, S* fout, int64_t blocks, int nq, int nspli
         if ((mode != 1 || !f64) && aligned) {
 #define VP_D(T, NT)                                                                                        \
     hipLaunchKernelGGL((drift_dma_kernel<S, R, PASSES, RHO, LANE_W, T, NT, VW>), dim3(blocks), dim3(kDriftThreads), \
-                       0, s, fin, fout, nq, nsplit, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial)
+                       0, s, fin, fout, nq, nsplit, na, groups < na * nsplit ? rot : 0, a_stride, row_stride,     \
+                       loop_stride, lt, W, IL, nvel, partial)
             if (mode == 3) {
                 if (nt) VP_D(3, true); else VP_D(3, false);
             } else {
@@ -1191,7 +1272,7 @@ static int launch_drift(const S* fin, S* fout, int64_t blocks, int nq, int nspli
         }
     }
     if constexpr (f64) {
-        hipLaunchKernelGGL((drift_kernel<R, PASSES, RHO, LANE_W>), dim3(blocks), dim3(kDriftThreads), 0, s, fin, fout,
+        hipLaunchKernelGGL((drift_kernel<R, PASSES, RHO, LANE_W>), dim3(unit_blocks), dim3(kDriftThreads), 0, s, fin, fout,
                            nq, nsplit, a_stride, row_stride, loop_stride, lt, W, IL, nvel, partial);
         return VP_OK;
     }
@@ -1211,7 +1292,6 @@ static int xdrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs&
         while (nvx * nq * S_ < 2 * cu_count() && S_ < static_cast<int>(ny)) S_ *= 2;
     }
     if (S_ > static_cast<int>(ny)) S_ = static_cast<int>(ny);
-    const int64_t blocks = nvx * nq * S_;
     const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
     const int nvel = static_cast<int>(nvx), lt = static_cast<int>(ny);
     int rc = VP_ERR_UNSUPPORTED;
@@ -1226,11 +1306,11 @@ static int xdrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs&
         constexpr int RV = (N) / KG > 0 ? (N) / KG : 1;                                                              \
         constexpr int RV2 = (N) / (2 * KG) > 0 ? (N) / (2 * KG) : 1;                                                 \
         if (pairs && (N) >= 64)                                                                                      \
-            rc = launch_drift<S, RV2, PV, false, false, 2>(fin, fout, blocks, nq, S_, a_stride, row_stride,          \
-                                                           loop_stride, lt, L.Wx, L.ILx, nvel, nullptr, s);          \
+            rc = launch_drift<S, RV2, PV, false, false, 2>(fin, fout, nq, S_, static_cast<int>(nvx), 0, 0, a_stride, \
+                                                           row_stride, loop_stride, lt, L.Wx, L.ILx, nvel, nullptr, s); \
         else                                                                                                         \
-            rc = launch_drift<S, RV, PV, false, false>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride, \
-                                                       lt, L.Wx, L.ILx, nvel, nullptr, s);                           \
+            rc = launch_drift<S, RV, PV, false, false>(fin, fout, nq, S_, static_cast<int>(nvx), 0, 0, a_stride,     \
+                                                       row_stride, loop_stride, lt, L.Wx, L.ILx, nvel, nullptr, s);  \
     }
     if (passes == 2) {
         VP_X(128, 2) else VP_X(64, 2) else VP_X(32, 2)
@@ -1250,9 +1330,9 @@ static int ydrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs&
     const int64_t nx = shape[0], ny = shape[1], nvx = shape[2], nvy = shape[3];
     if (nvy % KC != 0) return VP_ERR_UNSUPPORTED;
     const int nq = static_cast<int>(nvy / KC);
-    const int S_ = y_split(nvx, static_cast<int>(sizeof(S)));
+    const YPlan plan = y_plan(nx, ny, nvx, nvy, static_cast<int>(sizeof(S)));
+    const int S_ = plan.nsplit;
     if (rho && nq * S_ != L.nparts) return VP_ERR_WORKSPACE;  // the layout was computed for another split
-    const int64_t blocks = nx * nq * S_;
     const int64_t a_stride = ny * nvx * nvy, row_stride = nvx * nvy, loop_stride = nvy;
     const int nvel = static_cast<int>(nvy), lt = static_cast<int>(nvx);
     int rc = VP_ERR_UNSUPPORTED;
@@ -1266,12 +1346,14 @@ static int ydrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs&
         constexpr int RV2 = (N) / (2 * KG) > 0 ? (N) / (2 * KG) : 1;                                             \
         if constexpr (sizeof(S) == 4 && (N) >= 64) {                                                             \
             if (pairs)                                                                                           \
-                rc = launch_drift<S, RV2, 1, RH, true, 2>(fin, fout, blocks, nq, S_, a_stride, row_stride,       \
-                                                          loop_stride, lt, L.Wy, L.ILy, nvel, L.partial, s);     \
+                rc = launch_drift<S, RV2, 1, RH, true, 2>(fin, fout, nq, S_, static_cast<int>(nx), plan.groups,  \
+                                                          plan.rot, a_stride, row_stride, loop_stride, lt, L.Wy, \
+                                                          L.ILy, nvel, L.partial, s);                            \
         }                                                                                                        \
         if (rc == VP_ERR_UNSUPPORTED)                                                                            \
-            rc = launch_drift<S, RV, 1, RH, true>(fin, fout, blocks, nq, S_, a_stride, row_stride, loop_stride,  \
-                                                  lt, L.Wy, L.ILy, nvel, L.partial, s);                          \
+            rc = launch_drift<S, RV, 1, RH, true>(fin, fout, nq, S_, static_cast<int>(nx), plan.groups, plan.rot, \
+                                                  a_stride, row_stride, loop_stride, lt, L.Wy, L.ILy, nvel,       \
+                                                  L.partial, s);                                                  \
     }
     if (rho) {
         VP_Y(128, true) else VP_Y(64, true) else VP_Y(32, true)

@@ -101,7 +101,7 @@ _SIGS = {
 
 KNOBS = {"x_split": 0, "y_split": 1, "kick_prefetch": 2, "kick_kernel": 3, "drift_dma": 4, "nontemporal": 5,
          "ens_xtile": 6, "ens_ktile": 7, "ens_wgs": 8, "seg_rows": 9, "seg_contig": 10, "f32_pairs": 11,
-         "poisson_dft": 12}
+         "poisson_dft": 12, "y_units": 13}
 
 _lib = None
 _lock = threading.Lock()

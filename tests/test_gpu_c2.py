@@ -12,7 +12,7 @@ a row reduction of 8 threads per row and 4 columns each, the vx loop split
   simulations/vlasov2d_2v/pytorch/src/vlasov2D2V.py:943-998) on the
   two-stream initial condition at shapes the oracle steps in seconds:
   array-equal with the field off, <= 1e-13 with it on, for the default vx
-  split and for one split.
+  split, for one split and for persistent Y-drift workgroups.
 * At the full 128^4 shape, where the oracle would take minutes per step,
   size-independent properties: the fused runner (X halves of neighbouring
   steps merged) equals the generic six-sweep path, mass is conserved, and
@@ -106,24 +106,31 @@ def test_ny128_step_matches_oracle(device, shape, physics_mode):
     ref = traj[0]
     exact = _exact_delta_rho(f0.cpu().numpy(), g, arrays) if physics_mode else None
     lib = L.lib()
-    old = lib.vp_tune_get(L.KNOBS["y_split"])
+    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("y_split", "y_units")}
     try:
-        for split in (0, 1):  # 0: the default split (3 at nvx = 32, 12 at nvx = 64 and 128)
+        # (units mode, split): the default (one workgroup per (x, vx range),
+        # 3 ranges at nvx = 32, 12 at 64 and 128), one range, and persistent
+        # workgroups walking several units (two each by default; 3 ranges:
+        # ranges of 10-11 tiles and one or two units per workgroup), with
+        # rotated starts, spread over the XCDs
+        for units, split in ((0, 0), (1, 1), (2, 0), (3, 3)):
+            assert lib.vp_tune_set(L.KNOBS["y_units"], units) == 0
             assert lib.vp_tune_set(L.KNOBS["y_split"], split) == 0
             out, drho, phi, Ex, Ey = v2.step_with_field(f0, DT, G.vx, G.vy, G.kx, G.ky, G.inv_ksq, G.dx, G.dy,
                                                         G.dvx, G.dvy, physics_mode)
             torch.cuda.synchronize()
             if physics_mode:
-                assert rel(out.cpu().numpy(), ref[0]) <= 1e-13, split
+                assert rel(out.cpu().numpy(), ref[0]) <= 1e-13, (units, split)
                 d = drho.cpu().numpy()
-                assert np.max(np.abs(d - ref[1])) <= 1e-13 * np.max(np.abs(ref[1] + 1.0)), split
-                assert np.max(np.abs(d - exact)) <= 2 * np.finfo(np.float64).eps, split
+                assert np.max(np.abs(d - ref[1])) <= 1e-13 * np.max(np.abs(ref[1] + 1.0)), (units, split)
+                assert np.max(np.abs(d - exact)) <= 2 * np.finfo(np.float64).eps, (units, split)
                 for o, r in zip((phi, Ex, Ey), ref[2:5]):
-                    assert rel(o.cpu().numpy(), r) <= 1e-12, split
+                    assert rel(o.cpu().numpy(), r) <= 1e-12, (units, split)
             else:
                 np.testing.assert_array_equal(out.cpu().numpy(), ref[0])
     finally:
-        lib.vp_tune_set(L.KNOBS["y_split"], old)
+        for k, v in old.items():
+            lib.vp_tune_set(L.KNOBS[k], v)
 
 
 def test_ny128_fused_trajectory_matches_oracle(device):

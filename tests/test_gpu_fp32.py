@@ -136,6 +136,39 @@ def test_fp32_xdrift_two_passes_equal_two_calls(device):
     assert torch.equal(one, gen)
 
 
+@pytest.mark.parametrize("shape", [(64, 128, 32, 128), (64, 128, 64, 64)])
+def test_fp32_y_drift_launch_modes_agree(device, shape):
+    """Float32 Y drifts over 128-row lines: persistent workgroups walking
+    several (x, vx range) units (the default; rotated starts, also spread
+    over the XCDs, also with uneven ranges) and one workgroup per unit give
+    the same f bit for bit with the field off; with it on the charge
+    density's float64 sums run in another order: <= 1e-6 after 3 steps."""
+    v2, runner, L = _mods()
+    G, f0 = _setup(shape, 7)
+    f = torch.from_numpy(f0).float().to(device)
+    lib = L.lib()
+    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("y_split", "y_units")}
+    free, field = [], []
+    try:
+        for units, split in ((1, 0), (0, 0), (3, 3), (2, 5)):
+            lib.vp_tune_set(L.KNOBS["y_units"], units)
+            lib.vp_tune_set(L.KNOBS["y_split"], split)
+            a = runner.Run2D(G, f, 0.1, physics_mode=False)
+            a.advance(3)
+            b = runner.Run2D(G, f, 0.1, physics_mode=True)
+            b.advance(3)
+            free.append(a.state.clone())
+            field.append(b.state.double().cpu().numpy())
+            del a, b
+    finally:
+        for k, v in old.items():
+            lib.vp_tune_set(L.KNOBS[k], v)
+    for i in range(1, len(free)):
+        assert torch.equal(free[0], free[i]), i
+        assert np.max(np.abs(field[i] - field[0])) / np.max(np.abs(field[0])) <= 1e-6, i
+    assert not torch.equal(free[0], f)
+
+
 @pytest.mark.parametrize("shape", [(128, 64, 32, 128), (64, 64, 32, 64)])
 @pytest.mark.parametrize("passes", [1, 2])
 def test_fp32_xdrift_column_pairs_equal_single_columns(device, shape, passes):
