@@ -97,8 +97,9 @@ def fp32_storage(grid, f0, steps: int, warmup: int, err_steps: int) -> dict:
     field-energy history after `err_steps` steps against the float64 path
     (which matches the float64 oracle to <= 1e-12 in the parity tests).
     Two storage forms: "plain" (float32 f; the reported variant) and
-    "deviation" (float32 g = f - feq, feq the (x, y) mean of f0 in float32:
-    ~20x smaller error, slower kick)."""
+    "deviation" (float32 g = f - feq, feq = fx(vx) fy(vy) the separable part
+    of the (x, y) mean of f0 with float64 profiles: rounding scales with the
+    perturbation g instead of f)."""
     import torch
 
     from vlasov_amd import runner
@@ -164,7 +165,9 @@ def fp32_storage(grid, f0, steps: int, warmup: int, err_steps: int) -> dict:
         **errors("plain"),
         "field_energy_last_fp64": float(a[4][-1]),
         "deviation_storage": {
-            "storage": "float32 g = f - feq(vx, vy), feq the (x, y) mean of f0 in float32; kick in float64",
+            "storage": "float32 g = f - feq(vx, vy), feq = fx(vx) fy(vy) the separable part of the (x, y) mean of "
+                       "f0 (float64 profiles); drifts and kick of g in float32, kick correction (Kvx fx)(Kvy fy) - "
+                       "fx fy in float64",
             "ms_per_step": wall["deviation"] * 1e3, "speedup_vs_fp64": wall["fp64"] / wall["deviation"],
             **errors("deviation")},
     }

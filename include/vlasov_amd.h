@@ -266,36 +266,39 @@ int vp_fused_kick_f32(const float* f_in, float* f_out, const int64_t shape[4], c
 
 /* ---------------------------------------------------------------------
  * FP32 deviation storage (no reference counterpart): f = feq(vx, vy) +
- * g(x, y, vx, vy) with feq the (x, y) mean of the initial f rounded to
- * float32 and g in float32.  feq is defined as those float32 values (every
- * use widens them exactly to float64), so storing it loses nothing.  feq is
- * constant in x and y, so the drifts leave it unchanged and run on g (the
- * vp_fused_*drift_f32 kernels); the kick evaluates feq + g in float64 and
- * stores the new deviation; the charge density is sum(g) dv + sum(feq) dv.
- * Storage rounding then scales with the perturbation |g| instead of |f|:
- * the field-energy history stays within ~1e-6 of the float64 one where
- * plain FP32 storage drifts to ~1e-5.  Shapes: those of vp_run_2d2v_f32
- * with nvy == 128.
+ * g(x, y, vx, vy) with g in float32 and a separable equilibrium
+ * feq(vx, vy) = fx(vx) fy(vy) (the float64 product) from float64 profiles:
+ * the rank-one part of the (x, y) mean of the initial f (fx = its sums over
+ * vy, fy = its sums over vx over its total; for a mean a(vx) b(vy), as for
+ * the reference's initial conditions, fx fy is that mean).  feq is constant
+ * in x and y, so the drifts leave it unchanged and run on g (the
+ * vp_fused_*drift_f32 kernels); the kick, linear, maps fx fy to
+ * (Kvx fx)(Kvy fy), so it stores float(K32(g) + (Kvx fx)(Kvy fy) - fx fy):
+ * the float32 kick of g plus a float64 correction from the two profiles'
+ * 1D kicks per (x, y) plane; the charge density is sum(g) dv + sum(feq) dv.
+ * Storage and arithmetic rounding then scale with the perturbation |g|
+ * instead of |f|.  Shapes: those of vp_run_2d2v_f32 with nvy == 128.
+ * feq arrays: nvx + nvy float64 values, fx then fy.
  *
- * vp_dev_split_f32: feq[m] = float(mean over the nx*ny planes of f)
- *   (compensated), g = float(f - feq), rho_shift[0] = ((sum feq) * dvx) *
- *   dvy - 1 (device scalar); ws of vp_dev_split_workspace(shape) bytes.
- * vp_dev_join_f32: f = feq + g into f64_out and/or (rounded) f32_out.
+ * vp_dev_split_f32: the profiles (compensated sums) into feq,
+ *   g = float(f - fx fy), rho_shift[0] = ((sum fx fy) * dvx) * dvy - 1
+ *   (device scalar); ws of vp_dev_split_workspace(shape) bytes.
+ * vp_dev_join_f32: f = fx fy + g into f64_out and/or (rounded) f32_out.
  * vp_run_2d2v_dev_f32: vp_run_2d2v_f32 on g (same arguments and ws layout).
  * vp_fused_kick_dev_f32 / vp_fused_poisson_dev_f32: the two pieces that
  *   differ from the vp_fused_*_f32 ones (the drifts and tables are shared).
  * --------------------------------------------------------------------- */
 size_t vp_dev_split_workspace(const int64_t shape[4]);
-int vp_dev_split_f32(const double* f, const int64_t shape[4], double dvx, double dvy, float* feq,
+int vp_dev_split_f32(const double* f, const int64_t shape[4], double dvx, double dvy, double* feq,
                      double* rho_shift, float* g, void* ws, size_t ws_bytes, vp_stream_t stream);
-int vp_dev_join_f32(const float* g, const float* feq, const int64_t shape[4], float* f32_out,
+int vp_dev_join_f32(const float* g, const double* feq, const int64_t shape[4], float* f32_out,
                     double* f64_out, vp_stream_t stream);
-int vp_run_2d2v_dev_f32(float* g, const float* feq, const double* rho_shift, const int64_t shape[4],
+int vp_run_2d2v_dev_f32(float* g, const double* feq, const double* rho_shift, const int64_t shape[4],
                         int nsteps, double dt, const double* vx, const double* vy, const double* kx,
                         const double* ky, const double* inv_ksq, double dx, double dy, double dvx,
                         double dvy, int physics_mode, double* Ex, double* Ey, double* delta_rho,
                         double* phi, double* energy, void* ws, size_t ws_bytes, vp_stream_t stream);
-int vp_fused_kick_dev_f32(const float* g_in, float* g_out, const int64_t shape[4], const float* feq,
+int vp_fused_kick_dev_f32(const float* g_in, float* g_out, const int64_t shape[4], const double* feq,
                           const double* Ex, const double* Ey, double dt, double dvx, double dvy,
                           vp_stream_t stream);
 int vp_fused_poisson_dev_f32(const int64_t shape[4], double dvx, double dvy, const double* rho_shift,

@@ -849,15 +849,17 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
 // the pair shares one 7-value read.  A wave's LDS row is written and read by
 // that wave only: the write is waited for (lgkmcnt) before the reads, no
 // workgroup barrier.
-// DEV (deviation storage, S = float): fin/fout hold g = f - feq with feq
-// [NVX][NVY] float32 (constant in x, y; feq is defined as those float32
-// values, so nothing is lost by their storage); the kick evaluates
-// h = feq + g in float64 with float64 weights and stores float(K(h) - feq).
-// feq (64 KB at 128 x 128) is read from L2 twice per row: with the new
-// window row and for the output row.
+// DEV (deviation storage, S = float): fin/fout hold g = f - feq with the
+// separable feq(vx, vy) = fx(vx) fy(vy) (feq = fx | fy, float64 profiles).
+// The kick is linear and maps fx fy to ax ay, ax = Kvx fx and ay = Kvy fy (the
+// plane's Vx kick acts on the vx profile, its Vy kick on the vy profile), so
+// K(feq + g) - feq = K(g) + (ax ay - fx fy): the float32 kick of g as in the
+// plain kernel, plus a float64 correction that needs per plane the 1D kicks
+// of the two profiles (ax and fx of every row in LDS, ay and fy of the
+// lane's columns in registers) and nothing from memory per element.
 template <typename S, int NVX, int WPP, int PF, bool NT, bool DEV, int NVY = 128>
 __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict__ fin, S* __restrict__ fout,
-                                                             const float* __restrict__ feq,
+                                                             const double* __restrict__ feq,
                                                              const double* __restrict__ Ex,
                                                              const double* __restrict__ Ey, double dt, double dvx,
                                                              double dvy) {
@@ -870,7 +872,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     static_assert(NVX % WPP == 0 && RPW % PF == 0, "rows per wave must be a multiple of PF");
     static_assert(!DEV || sizeof(S) == 4, "deviation storage is float32");
     static_assert(!DEV || CPL == 2, "deviation storage: 128 vy columns");
-    typedef typename std::conditional<DEV, double, S>::type C;  // arithmetic type
+    typedef S C;  // arithmetic type
     typedef S V2s_ __attribute__((ext_vector_type(2)));
     typedef C V2_ __attribute__((ext_vector_type(2)));
     typedef typename std::conditional<CPL == 2, V2s_, S>::type V2s;  // a lane's part of a row, stored
@@ -879,6 +881,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     __shared__ int ilx[NVX];
     __shared__ int irregular[1];
     __shared__ C rowbuf[WPP][2][NVY + 2 * PAD];
+    __shared__ double axs[DEV ? NVX : 1], fxs[DEV ? NVX : 1];  // DEV: Kvx fx and fx of every row
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;
@@ -894,7 +897,19 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
         round_weights<C>(w, wc);
 #pragma unroll
         for (int k = 0; k < 6; ++k) wx[r * 6 + k] = wc[k];
-        ilx[r] = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
+        const int l = static_cast<int>(d.il < -8 ? -8 : (d.il > NVX + 8 ? NVX + 8 : d.il));
+        ilx[r] = l;
+        if constexpr (DEV) {
+            // Vx kick of the vx profile (taps outside the plane read zero)
+            double v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int j = l + k - 2;
+                v[k] = (j >= 0 && j < NVX) ? feq[j] : 0.0;
+            }
+            axs[r] = tap_sum<double>(w, v);
+            fxs[r] = feq[r];
+        }
     }
     // zero pads of the wave's two row buffers: 2 x 2 x PAD = 64 entries, one per lane
     {
@@ -903,6 +918,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     }
     C wy[CPL][6];
     int ty[CPL];  // LDS index of tap 0 of each column
+    double ay[DEV ? CPL : 1], fy[DEV ? CPL : 1];  // DEV: Kvy fy and fy of the lane's columns
     {
         const double ey = Ey[p];
 #pragma unroll
@@ -913,6 +929,17 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             round_weights<C>(w, wy[j]);
             const int l = static_cast<int>(d.il < -8 ? -8 : (d.il > NVY + 8 ? NVY + 8 : d.il));
             ty[j] = PAD + l - 2;
+            if constexpr (DEV) {
+                const double* fyp = feq + NVX;
+                double v[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const int i = l + k - 2;
+                    v[k] = (i >= 0 && i < NVY) ? fyp[i] : 0.0;
+                }
+                ay[j] = tap_sum<double>(w, v);
+                fy[j] = fyp[CPL * lane + j];
+            }
         }
     }
     bool pair = false;
@@ -926,36 +953,22 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
 
     const S* src = fin + p * NVX * NVY + CPL * lane;
     S* dst = fout + p * NVX * NVY + CPL * lane;
-    const float* eq = DEV ? feq + 2 * lane : nullptr;
-    typedef float V2f __attribute__((ext_vector_type(2)));
-    // A row as loaded (g, and feq for DEV) and as used (h = feq + g, or f).
+    // A row as loaded and as used (the same values).
     struct Raw {
         V2s g;
-        V2f e;
     };
     auto ld = [&](int row) -> Raw {
         Raw r;
         const V2s* a = reinterpret_cast<const V2s*>(src + row * NVY);
         r.g = NT ? __builtin_nontemporal_load(a) : *a;
-        if constexpr (DEV) r.e = *reinterpret_cast<const V2f*>(eq + row * NVY);
         return r;
     };
-    auto val = [](const Raw& r) -> V2 {
-        V2 v;
-        if constexpr (DEV) {
-            v.x = static_cast<double>(r.e.x) + static_cast<double>(r.g.x);
-            v.y = static_cast<double>(r.e.y) + static_cast<double>(r.g.y);
-        } else {
-            v = r.g;
-        }
-        return v;
-    };
-    (void)val;
+    auto val = [](const Raw& r) -> V2 { return r.g; };
     auto clampr = [](int j) { return j < 0 ? 0 : (j > NVX - 1 ? NVX - 1 : j); };
     auto maskr = [](int j) { return (j >= 0 && j < NVX) ? C(1) : C(0); };
-    // Vy kick of row r from the Vx results ox of the lane's two columns;
-    // eo: feq of row r at those columns (DEV).
-    auto vy_row = [&](int r, C ox0, C ox1, V2f eo) {
+    // Vy kick of row r from the Vx results ox of the lane's two columns
+    // (DEV: plus the correction of row r).
+    auto vy_row = [&](int r, C ox0, C ox1) {
         C* rb = rowbuf[wv][r & 1];
         if constexpr (CPL == 1) {
             rb[PAD + lane] = ox0;
@@ -970,7 +983,6 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             else
                 *out = res;
             (void)ox1;
-            (void)eo;
         } else {
             V2_ o;
             o.x = ox0;
@@ -996,8 +1008,9 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             }
             V2s res;
             if constexpr (DEV) {
-                res.x = static_cast<float>(tap_sum<double>(wy[0], a) - static_cast<double>(eo.x));
-                res.y = static_cast<float>(tap_sum<double>(wy[1], b) - static_cast<double>(eo.y));
+                const double axr = axs[r], fxr = fxs[r];
+                res.x = static_cast<float>(static_cast<double>(tap_sum<C>(wy[0], a)) + (axr * ay[0] - fxr * fy[0]));
+                res.y = static_cast<float>(static_cast<double>(tap_sum<C>(wy[1], b)) + (axr * ay[1] - fxr * fy[1]));
             } else {
                 res.x = tap_sum<C>(wy[0], a);
                 res.y = tap_sum<C>(wy[1], b);
@@ -1008,12 +1021,6 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             else
                 *out = res;
         }
-    };
-    // feq of output row r (DEV; an on-chip read: the 64 KB table stays in L2)
-    auto eq_row = [&](int r) -> V2f {
-        V2f e = {0.0f, 0.0f};
-        if constexpr (DEV) e = *reinterpret_cast<const V2f*>(eq + r * NVY);
-        return e;
     };
     auto vx_pair = [&](int r, const V2 v[6], C& ox0, C& ox1) {
         C w[6], a[6], b[6];
@@ -1051,7 +1058,6 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 const int jn = r + s0 + 3;  // row held by q[u]: tap 5 of row r
                 const V2 v5 = val(q[u]) * maskr(jn);
                 const V2 v[6] = {win[0], win[1], win[2], win[3], win[4], v5};
-                const V2f eo = eq_row(r);
                 C ox0, ox1;
                 vx_pair(r, v, ox0, ox1);
                 win[0] = win[1];
@@ -1060,7 +1066,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 win[3] = win[4];
                 win[4] = v5;
                 if (i0 + u + PF < RPW) q[u] = ld(clampr(jn + PF));
-                vy_row(r, ox0, ox1, eo);
+                vy_row(r, ox0, ox1);
             }
         }
     } else if (regular) {
@@ -1079,7 +1085,6 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 const int jn = r + s0 - 2;  // row held by q[u]: tap 0 of row r
                 const V2 v0 = val(q[u]) * maskr(jn);
                 const V2 v[6] = {v0, win[0], win[1], win[2], win[3], win[4]};
-                const V2f eo = eq_row(r);
                 C ox0, ox1;
                 vx_pair(r, v, ox0, ox1);
                 win[4] = win[3];
@@ -1088,7 +1093,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 win[1] = win[0];
                 win[0] = v0;
                 if (i0 + u + PF < RPW) q[u] = ld(clampr(jn - PF));
-                vy_row(r, ox0, ox1, eo);
+                vy_row(r, ox0, ox1);
             }
         }
     } else {
@@ -1098,10 +1103,9 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
             V2 v[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) v[k] = val(ld(clampr(l + k - 2))) * maskr(l + k - 2);
-            const V2f eo = eq_row(r);
             C ox0, ox1;
             vx_pair(r, v, ox0, ox1);
-            vy_row(r, ox0, ox1, eo);
+            vy_row(r, ox0, ox1);
         }
     }
 }
@@ -1429,7 +1433,7 @@ static int launch_kick_stream(const S* fin, S* fout, int nplanes, const double* 
 // prefetched (else 4), at most the rows per wave; feq != nullptr selects
 // deviation storage (float only).
 template <typename S, int NVX, int WPP, int NVY>
-static int launch_kick_rows(bool pf8, const S* fin, S* fout, const float* feq, int nplanes, const double* Ex,
+static int launch_kick_rows(bool pf8, const S* fin, S* fout, const double* feq, int nplanes, const double* Ex,
                             const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
     constexpr int RPW = NVX / WPP;
     constexpr int PF8 = RPW < 8 ? RPW : 8;
@@ -1461,14 +1465,15 @@ static int launch_kick_rows(bool pf8, const S* fin, S* fout, const float* feq, i
 // plane, 2 the two-wave row-streaming kernel; the kick-prefetch knob 1 or 3
 // picks 4 or 8 rows prefetched.  Automatic (knobs 0 and 2): 8 waves per
 // plane, 4 rows prefetched for double and 8 for float (measured fastest at
-// 128^4).  Deviation storage always takes a row-split kernel: automatic = 2
-// waves per plane and 4 rows prefetched (float64 arithmetic: ~125 registers).
+// 128^4).  Deviation storage takes a row-split kernel, automatic = 4 waves
+// per plane and 8 rows prefetched (at 128^4: 0.39 ms against 0.41 with 2
+// waves and 0.58 with 8: its per-plane start reads the profiles' taps).
 template <typename S, int NVX, int NVY = 128>
-static int kick_rows_dispatch(int kk, int pfk, const S* fin, S* fout, const float* feq, int nplanes,
+static int kick_rows_dispatch(int kk, int pfk, const S* fin, S* fout, const double* feq, int nplanes,
                               const double* Ex, const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
-    if (feq && (kk < 3 || kk > 6)) kk = 3;
+    if (feq && (kk < 3 || kk > 6)) kk = 4;
     if (kk == 0) kk = 5;
-    const bool pf8 = pfk == 3 || (pfk != 1 && sizeof(S) == 4 && !feq);
+    const bool pf8 = pfk == 3 || (pfk != 1 && sizeof(S) == 4);
 #define VP_R(W) return launch_kick_rows<S, NVX, W, NVY>(pf8, fin, fout, feq, nplanes, Ex, Ey, dt, dvx, dvy, s)
     if (kk == 3) VP_R(2);
     if (kk == 4) VP_R(4);
@@ -1542,9 +1547,10 @@ int fast2d_kick(const double* fin, double* fout, const int64_t shape[4], const d
     return VP_ERR_UNSUPPORTED;
 }
 
-// FP32 deviation storage: g = f - feq in float32 (feq float32), kick evaluated in float64
+// FP32 deviation storage: g = f - feq in float32 with the separable feq = fx fy
+// (feq = fx | fy, float64): the float32 kick of g plus a float64 correction
 // (row-split kernel; 128 vy columns; g_in and g_out must not overlap).
-int fast2d_kick_dev(const float* gin, float* gout, const int64_t shape[4], const float* feq, const double* Ex,
+int fast2d_kick_dev(const float* gin, float* gout, const int64_t shape[4], const double* feq, const double* Ex,
                     const double* Ey, double dt, double dvx, double dvy, hipStream_t s) {
     const int64_t nvx = shape[2], nvy = shape[3];
     const int64_t cells = shape[0] * shape[1] * nvx * nvy;

@@ -42,9 +42,10 @@ int fast2d_xdrift(const float* fin, float* fout, const int64_t shape[4], const F
                   hipStream_t s);
 int fast2d_ydrift(const float* fin, float* fout, const int64_t shape[4], const Fast2DWs& L, bool rho,
                   hipStream_t s);
-// Deviation storage (float32 g = f - feq, feq [nvx][nvy] float32): kick of
-// feq + g in float64, stores the new deviation; nvy == 128, out of place.
-int fast2d_kick_dev(const float* gin, float* gout, const int64_t shape[4], const float* feq, const double* Ex,
+// Deviation storage (float32 g = f - feq, feq = fx | fy: float64 profiles of
+// the separable feq(vx, vy) = fx(vx) fy(vy)): the float32 kick of g plus the
+// float64 correction (Kvx fx)(Kvy fy) - fx fy; nvy == 128, out of place.
+int fast2d_kick_dev(const float* gin, float* gout, const int64_t shape[4], const double* feq, const double* Ex,
                     const double* Ey, double dt, double dvx, double dvy, hipStream_t s);
 int fast2d_kick(const float* fin, float* fout, const int64_t shape[4], const double* Ex, const double* Ey, double dt,
                 double dvx, double dvy, hipStream_t s);

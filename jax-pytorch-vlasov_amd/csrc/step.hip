@@ -311,7 +311,7 @@ template <typename S>
 static int run_2d2v(S* f, const int64_t shape[4], int nsteps, double dt, const double* vx, const double* vy,
                     const double* kx, const double* ky, const double* inv_ksq, double dx, double dy, double dvx,
                     double dvy, int physics_mode, double* Ex, double* Ey, double* delta_rho, double* phi,
-                    double* energy, void* ws, hipStream_t s, const float* feq = nullptr,
+                    double* energy, void* ws, hipStream_t s, const double* feq = nullptr,
                     const double* rho_shift = nullptr) {
     const int64_t nx = shape[0], ny = shape[1];
     vp::Fast2DWs L = vp::fast2d_layout(shape, ws, static_cast<int>(sizeof(S)), true);
@@ -374,13 +374,14 @@ extern "C" int vp_run_2d2v_f32(float* f, const int64_t shape[4], int nsteps, dou
                            delta_rho, phi, energy, ws, static_cast<hipStream_t>(stream));
 }
 
-// FP32 deviation storage: g (float32) = f - feq, feq [nvx][nvy] float32,
-// rho_shift[0] = sum feq dvx dvy - 1 (both from vp_dev_split_f32).
+// FP32 deviation storage: g (float32) = f - feq, feq = fx | fy (float64
+// profiles of the separable feq = fx(vx) fy(vy)), rho_shift[0] = sum feq dvx
+// dvy - 1 (all from vp_dev_split_f32).
 static bool dev_supported(const int64_t shape[4]) {
     return shape && vp::fast2d_supported_f32(shape) && shape[3] == 128;
 }
 
-extern "C" int vp_run_2d2v_dev_f32(float* g, const float* feq, const double* rho_shift, const int64_t shape[4],
+extern "C" int vp_run_2d2v_dev_f32(float* g, const double* feq, const double* rho_shift, const int64_t shape[4],
                                    int nsteps, double dt, const double* vx, const double* vy, const double* kx,
                                    const double* ky, const double* inv_ksq, double dx, double dy, double dvx,
                                    double dvy, int physics_mode, double* Ex, double* Ey, double* delta_rho,
@@ -403,7 +404,7 @@ extern "C" int vp_fused_poisson_dev_f32(const int64_t shape[4], double dvx, doub
                               delta_rho, phi, L.pws, L.pws_bytes, static_cast<hipStream_t>(stream), rho_shift);
 }
 
-extern "C" int vp_fused_kick_dev_f32(const float* g_in, float* g_out, const int64_t shape[4], const float* feq,
+extern "C" int vp_fused_kick_dev_f32(const float* g_in, float* g_out, const int64_t shape[4], const double* feq,
                                      const double* Ex, const double* Ey, double dt, double dvx, double dvy,
                                      vp_stream_t stream) {
     VP_RETURN_IF(!shape || !g_in || !g_out || !feq || !Ex || !Ey, VP_ERR_ARG);

@@ -15,9 +15,11 @@ For float64 f both paths give the same f as repeated calls of
 ``vlasov_step_impl``.  float32 f is the FP32-storage variant: f stored and
 interpolated in float32, charge density, Poisson solve, fields and field
 energy in float64.  ``storage="deviation"`` (fused, nvy = 128) stores
-g = f - feq in float32 instead, feq the (x, y) mean of the initial
-f: the drifts run on g, the kick on feq + g in float64, and storage
-rounding scales with the perturbation instead of with f.
+g = f - feq in float32 instead, feq(vx, vy) = fx(vx) fy(vy) the separable
+(rank-one) part of the (x, y) mean of the initial f with float64 profiles:
+the drifts run on g, the kick on g in float32 plus the float64 correction
+(Kvx fx)(Kvy fy) - fx fy, and rounding scales with the perturbation instead
+of with f.
 """
 from __future__ import annotations
 
@@ -79,10 +81,11 @@ class Run2D:
                 raise ValueError(f"no fused kernels for shape {self.shape} and {f.dtype}")
         Nx, Ny = self.shape[0], self.shape[1]
         if self.deviation:
-            # split on the device: feq = (x, y) mean of f (float64), g = float32(f - feq)
+            # split on the device: feq = fx fy from the (x, y) mean of f, g = float32(f - feq);
+            # self.feq holds the float64 profiles fx (nvx values) then fy (nvy values)
             f64 = f.contiguous() if f.dtype == torch.float64 else f.contiguous().to(torch.float64)
             sh = _lib.i64x4(self.shape)
-            self.feq = torch.empty(self.shape[2:], dtype=torch.float32, device=dev)
+            self.feq = torch.empty(self.shape[2] + self.shape[3], dtype=torch.float64, device=dev)
             self.rho_shift = torch.empty(1, dtype=torch.float64, device=dev)
             g = torch.empty(self.shape, dtype=torch.float32, device=dev)
             sws = torch.empty(lib().vp_dev_split_workspace(sh), dtype=torch.uint8, device=dev)

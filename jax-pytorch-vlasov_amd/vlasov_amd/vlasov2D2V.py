@@ -417,9 +417,10 @@ class Vlasov:
         charge density, Poisson solve and fields stay float64 (the
         reference's float32 mode computes everything in float32).
         ``fp32_storage`` (float32 only): "plain" stores f itself; "deviation"
-        stores g = f - feq in float32, feq the (x, y) mean of the initial f in
-        float32 (used when the shape has those kernels: nvy = 128), which cuts
-        the storage error of the field energy ~20x for a slower kick.
+        stores g = f - feq in float32, feq = fx(vx) fy(vy) the separable part
+        of the (x, y) mean of the initial f with float64 profiles (used when
+        the shape has those kernels: nvy = 128), so that rounding scales with
+        the perturbation g instead of f.
         """
         self.output_log = ""
         if dtype not in ("float64", "float32"):

@@ -282,13 +282,15 @@ def test_fp32_kick_kernels_agree(device, shape):
 
 
 # ---------------------------------------------------------------------------
-# Deviation storage: float32 g = f - feq, feq the float64 (x, y) mean of f.
+# Deviation storage: float32 g = f - feq, feq = fx(vx) fy(vy) from float64
+# profiles of the (x, y) mean of f (fx: its sums over vy, fy: its sums over
+# vx over its total).
 # ---------------------------------------------------------------------------
 def _split(L, f64, shape, dvx, dvy):
     lib = L.lib()
     dev = f64.device
     sh = L.i64x4(shape)
-    feq = torch.empty(shape[2:], dtype=torch.float32, device=dev)
+    feq = torch.empty(shape[2] + shape[3], dtype=torch.float64, device=dev)
     rs = torch.empty(1, dtype=torch.float64, device=dev)
     g = torch.empty(shape, dtype=torch.float32, device=dev)
     ws = torch.empty(lib.vp_dev_split_workspace(sh), dtype=torch.uint8, device=dev)
@@ -301,64 +303,97 @@ def _split(L, f64, shape, dvx, dvy):
 def test_deviation_split_and_join(device):
     v2, runner, L = _mods()
     shape = (8, 6, 32, 128)
+    nvx = shape[2]
     rng = np.random.default_rng(21)
     f = rng.random(shape) + 1.0
     dvx, dvy = 0.3, 0.2
     feq, rs, g = _split(L, torch.from_numpy(f).to(device), shape, dvx, dvy)
-    ref_feq = f.reshape(-1, shape[2] * shape[3]).mean(axis=0).reshape(shape[2:])
-    e = feq.cpu().numpy().astype(np.float64)  # feq is defined as these float32 values
-    np.testing.assert_allclose(e, ref_feq, rtol=2.0 ** -24)
-    assert abs(float(rs) - ((e.sum() * dvx) * dvy - 1.0)) <= 1e-12 * e.sum() * dvx * dvy
-    np.testing.assert_array_equal(g.cpu().numpy(), (f - e[None, None]).astype(np.float32))
+    M = f.reshape(-1, shape[2] * shape[3]).mean(axis=0).reshape(shape[2:])
+    prof = feq.cpu().numpy()
+    fx, fy = prof[:nvx], prof[nvx:]
+    np.testing.assert_allclose(fx, M.sum(axis=1), rtol=1e-14)
+    np.testing.assert_allclose(fy, M.sum(axis=0) / M.sum(), rtol=1e-14)
+    E = fx[:, None] * fy[None, :]  # feq as every kernel forms it (one rounded product)
+    assert abs(float(rs) - ((E.sum() * dvx) * dvy - 1.0)) <= 1e-12 * E.sum() * dvx * dvy
+    np.testing.assert_array_equal(g.cpu().numpy(), (f - E[None, None]).astype(np.float32))
     out32 = torch.empty(shape, dtype=torch.float32, device=device)
     out64 = torch.empty(shape, dtype=torch.float64, device=device)
     L.check(L.lib().vp_dev_join_f32(g.data_ptr(), feq.data_ptr(), L.i64x4(shape), out32.data_ptr(), out64.data_ptr(),
                                     torch.cuda.current_stream().cuda_stream), "vp_dev_join_f32")
-    h = feq.cpu().numpy().astype(np.float64)[None, None] + g.cpu().numpy().astype(np.float64)
+    h = E[None, None] + g.cpu().numpy().astype(np.float64)
     np.testing.assert_array_equal(out64.cpu().numpy(), h)
     np.testing.assert_array_equal(out32.cpu().numpy(), h.astype(np.float32))
     # the rounding of the stored deviation only: |f - join| <= half an ulp of g
-    assert np.max(np.abs(out64.cpu().numpy() - f)) <= np.max(np.abs(f - e[None, None])) * 2.0 ** -24 * 1.01
+    assert np.max(np.abs(out64.cpu().numpy() - f)) <= np.max(np.abs(f - E[None, None])) * 2.0 ** -24 * 1.01
+
+
+def test_deviation_profiles_of_a_separable_mean(device):
+    """A mean a(vx) b(vy) (every initial condition of the reference) is its
+    own rank-one part: fx fy reproduces it to rounding, so g holds only the
+    (x, y) perturbation."""
+    v2, runner, L = _mods()
+    shape = (8, 4, 32, 128)
+    x = np.arange(shape[0]) / shape[0] * 2 * math.pi
+    vx = np.linspace(-8, 8, shape[2])
+    vy = np.linspace(-8, 8, shape[3])
+    a = 0.5 * (np.exp(-0.5 * (vx - 2.4) ** 2) + np.exp(-0.5 * (vx + 2.4) ** 2))
+    b = np.exp(-0.5 * vy ** 2) / math.sqrt(2 * math.pi)
+    pert = 1 + 0.001 * np.cos(x)
+    f = pert[:, None, None, None] * np.ones(shape[1])[None, :, None, None] * (a[:, None] * b[None, :])[None, None]
+    feq, rs, g = _split(L, torch.from_numpy(f).to(device), shape, 0.1, 0.1)
+    prof = feq.cpu().numpy()
+    E = prof[:shape[2], None] * prof[None, shape[2]:]
+    M = f.mean(axis=(0, 1))
+    assert np.max(np.abs(E - M)) <= 4e-15 * np.max(M)
+    gg = g.cpu().numpy().astype(np.float64)
+    dev_ref = f - M[None, None]
+    assert np.max(np.abs(gg - dev_ref)) <= 1e-6 * np.max(np.abs(dev_ref)) + 1e-15 * np.max(M)
 
 
 @pytest.mark.parametrize("shape", [(4, 4, 128, 128), (4, 8, 32, 128)])
-def test_deviation_kick_equals_float64_kick_of_the_sum(device, shape):
-    """The deviation kick stores float32(K(feq + g) - feq) with K the float64
-    kick: bit for bit the float64 oracle kick of the same feq + g, for every
-    launch shape (the output feq from the Vx window or read; multi-cell and
-    out-of-domain shifts)."""
+def test_deviation_kick_matches_float64_kick_of_the_sum(device, shape):
+    """The deviation kick stores float32(K32(g) + (Kvx fx)(Kvy fy) - fx fy):
+    against the float64 oracle kick of fx fy + g minus fx fy, within the
+    float32 arithmetic of the kick of g (64 float32 ulps of max |g|) for
+    every launch shape, with multi-cell and out-of-domain shifts; with g = 0
+    the correction alone, float64 but for the final rounding."""
     v2, runner, L = _mods()
     lib = L.lib()
     nx, ny, nvx, nvy = shape
     rng = np.random.default_rng(8)
-    feq = np.exp(-0.5 * np.linspace(-4, 4, nvx)[:, None] ** 2
-                 - 0.5 * np.linspace(-4, 4, nvy)[None, :] ** 2).astype(np.float32)
-    g = (rng.random(shape) - 0.5).astype(np.float32) * np.float32(1e-3)
+    fx = np.exp(-0.5 * np.linspace(-4, 4, nvx) ** 2)
+    fy = np.exp(-0.5 * np.linspace(-4, 4, nvy) ** 2) / 2.5
+    E = fx[:, None] * fy[None, :]
     dt, dvx, dvy = 0.1, 8.0 / (nvx - 1), 8.0 / (nvy - 1)
     choices = np.array([0.0, 1e-18, 0.4, -0.7, 2.0, 5.3, -5.3, 80.0, -80.0])
     Ex = rng.choice(choices, size=(nx, ny)) * dvx / dt
     Ey = rng.choice(choices, size=(nx, ny)) * dvy / dt
-    h = feq.astype(np.float64)[None, None] + g.astype(np.float64)
-    ref = vo.advect_axis(h, Ex[:, :, None], 2, dt, dvx, periodic=False)
-    ref = vo.advect_axis(ref, Ey[:, :, None], 3, dt, dvy, periodic=False)
-    expect = (ref - feq.astype(np.float64)[None, None]).astype(np.float32)
-    tg = torch.from_numpy(g).to(device)
-    out = torch.empty_like(tg)
     s = torch.cuda.current_stream().cuda_stream
-    args = (L.i64x4(shape), torch.from_numpy(feq).to(device), torch.from_numpy(Ex).to(device),
-            torch.from_numpy(Ey).to(device))
-    old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("kick_kernel", "kick_prefetch")}
-    try:
-        for kk, pf in ((0, 2), (0, 1), (4, 3), (3, 1)):
-            lib.vp_tune_set(L.KNOBS["kick_kernel"], kk)
-            lib.vp_tune_set(L.KNOBS["kick_prefetch"], pf)
-            L.check(lib.vp_fused_kick_dev_f32(tg.data_ptr(), out.data_ptr(), args[0], args[1].data_ptr(),
-                                              args[2].data_ptr(), args[3].data_ptr(), dt, dvx, dvy, s),
-                    "vp_fused_kick_dev_f32")
-            np.testing.assert_array_equal(out.cpu().numpy(), expect)
-    finally:
-        for k, v in old.items():
-            lib.vp_tune_set(L.KNOBS[k], v)
+    prof = torch.from_numpy(np.concatenate([fx, fy])).to(device)
+    tEx, tEy = torch.from_numpy(Ex).to(device), torch.from_numpy(Ey).to(device)
+    for scale in (1e-3, 0.0):
+        g = ((rng.random(shape) - 0.5) * scale).astype(np.float32)
+        h = E[None, None] + g.astype(np.float64)
+        ref = vo.advect_axis(h, Ex[:, :, None], 2, dt, dvx, periodic=False)
+        ref = vo.advect_axis(ref, Ey[:, :, None], 3, dt, dvy, periodic=False)
+        expect = ref - E[None, None]
+        tg = torch.from_numpy(g).to(device)
+        out = torch.empty_like(tg)
+        old = {k: lib.vp_tune_get(L.KNOBS[k]) for k in ("kick_kernel", "kick_prefetch")}
+        try:
+            for kk, pf in ((0, 2), (0, 1), (5, 3), (3, 1), (6, 0)):
+                lib.vp_tune_set(L.KNOBS["kick_kernel"], kk)
+                lib.vp_tune_set(L.KNOBS["kick_prefetch"], pf)
+                L.check(lib.vp_fused_kick_dev_f32(tg.data_ptr(), out.data_ptr(), L.i64x4(shape), prof.data_ptr(),
+                                                  tEx.data_ptr(), tEy.data_ptr(), dt, dvx, dvy, s),
+                        "vp_fused_kick_dev_f32")
+                o = out.cpu().numpy().astype(np.float64)
+                ulp = np.abs(expect).astype(np.float32).astype(np.float64) * 2.0 ** -23
+                tol = 64 * 2.0 ** -24 * float(np.max(np.abs(g))) + 1e-15 * np.max(E)
+                assert np.all(np.abs(o - expect) <= tol + ulp), (scale, kk, pf, float(np.max(np.abs(o - expect))))
+        finally:
+            for k, v in old.items():
+                lib.vp_tune_set(L.KNOBS[k], v)
 
 
 def test_deviation_trajectory_vs_fp64_oracle(device):
