@@ -96,10 +96,11 @@ def fp32_storage(grid, f0, steps: int, warmup: int, err_steps: int) -> dict:
     no per-kernel events) and the error of f, phi, Ex, Ey and the
     field-energy history after `err_steps` steps against the float64 path
     (which matches the float64 oracle to <= 1e-12 in the parity tests).
-    Two storage forms: "plain" (float32 f; the reported variant) and
-    "deviation" (float32 g = f - feq, feq = fx(vx) fy(vy) the separable part
-    of the (x, y) mean of f0 with float64 profiles: rounding scales with the
-    perturbation g instead of f)."""
+    Two storage forms: "plain" (float32 f) and "deviation" (float32
+    g = f - feq, feq = fx(vx) fy(vy) the separable part of the (x, y) mean of
+    f0 with float64 profiles: rounding scales with the perturbation g
+    instead of f).  The record's top level reports the form that meets the
+    target (speed-up >= 1.8, masked pointwise field-energy error <= 1e-5)."""
     import torch
 
     from vlasov_amd import runner
@@ -154,22 +155,32 @@ def fp32_storage(grid, f0, steps: int, warmup: int, err_steps: int) -> dict:
             "steps_masked": int(mask.sum()),
         }
 
+    plain = {"storage": "float32 f (weights rounded to float32 with a unit sum); rho, Poisson, fields in float64",
+             "ms_per_step": wall["plain"] * 1e3, "speedup_vs_fp64": wall["fp64"] / wall["plain"], **errors("plain")}
+    dev = {"storage": "float32 g = f - feq(vx, vy), feq = fx(vx) fy(vy) the separable part of the (x, y) mean of "
+                      "f0 (float64 profiles); drifts and kick of g in float32, kick correction (Kvx fx)(Kvy fy) - "
+                      "fx fy in float64",
+           "ms_per_step": wall["deviation"] * 1e3, "speedup_vs_fp64": wall["fp64"] / wall["deviation"],
+           **errors("deviation")}
+    for form in (plain, dev):
+        form["meets_target"] = bool(form["speedup_vs_fp64"] >= 1.8 and form["max_rel_err_field_energy_masked"] <= 1e-5)
+    # the reported form: one that meets the target (the faster if both do), else the more accurate
+    best = max((f for f in (dev, plain) if f["meets_target"]), key=lambda f: f["speedup_vs_fp64"], default=dev)
+    name = "deviation" if best is dev else "plain"
     res = {
-        "storage": "float32 f (weights rounded to float32 with a unit sum); rho, Poisson, fields in float64",
-        "ms_per_step": wall["plain"] * 1e3,
-        "value": grid.Nx * grid.Ny * grid.Nvx * grid.Nvy / wall["plain"],
+        "form": name,
+        "target": "speed-up >= 1.8 over the FP64 step with <= 1e-5 field-energy error, pointwise at every step "
+                  "where W_E(fp64) > 1e-6 max W_E(fp64), over the error steps",
+        "ms_per_step": best["ms_per_step"],
+        "value": grid.Nx * grid.Ny * grid.Nvx * grid.Nvy / (best["ms_per_step"] * 1e-3),
         "fp64_ms_per_step_same_timing": wall["fp64"] * 1e3,
-        "speedup_vs_fp64": wall["fp64"] / wall["plain"],
+        "speedup_vs_fp64": best["speedup_vs_fp64"],
+        "max_rel_err_field_energy_masked": best["max_rel_err_field_energy_masked"],
+        "meets_target": best["meets_target"],
         "error_steps": err_steps,
         "error_reference": "float64 path of this library, same initial f, same steps",
-        **errors("plain"),
         "field_energy_last_fp64": float(a[4][-1]),
-        "deviation_storage": {
-            "storage": "float32 g = f - feq(vx, vy), feq = fx(vx) fy(vy) the separable part of the (x, y) mean of "
-                       "f0 (float64 profiles); drifts and kick of g in float32, kick correction (Kvx fx)(Kvy fy) - "
-                       "fx fy in float64",
-            "ms_per_step": wall["deviation"] * 1e3, "speedup_vs_fp64": wall["fp64"] / wall["deviation"],
-            **errors("deviation")},
+        "forms": {"deviation": dev, "plain": plain},
     }
     return res
 
@@ -441,17 +452,20 @@ FP64_KERNELS = {
 }
 
 
+# Sources the profiled kernels (the fused float64 sweeps and the copy
+# calibration kernel) are built from, launch knobs included.
+PROFILED_SOURCES = ("jax-pytorch-vlasov_amd/csrc/fast2d.hip", "jax-pytorch-vlasov_amd/csrc/fast2d.hpp",
+                    "jax-pytorch-vlasov_amd/csrc/vp_common.hpp", "jax-pytorch-vlasov_amd/csrc/stream.hip",
+                    "include/vlasov_amd.h")
+
+
 def kernel_sources_sha() -> str:
-    """sha256 over the library's kernel sources and C header (sorted by name):
-    a counter profile is valid only for the sources it was taken from."""
-    import glob
+    """sha256 over the sources of the profiled kernels: a counter profile is
+    valid only for the sources it was taken from."""
     import hashlib
 
     h = hashlib.sha256()
-    pkg = os.path.join(ROOT, "jax-pytorch-vlasov_amd", "csrc")
-    files = sorted(glob.glob(os.path.join(pkg, "*.hip")) + glob.glob(os.path.join(pkg, "*.hpp")) +
-                   glob.glob(os.path.join(ROOT, "include", "*.h")))
-    for path in files:
+    for path in sorted(os.path.join(ROOT, p) for p in PROFILED_SOURCES):
         h.update(os.path.basename(path).encode())
         with open(path, "rb") as fh:
             h.update(fh.read())
@@ -508,7 +522,14 @@ def summarize_counters(root_dir: str, n: int) -> dict:
 
     ff, wf = copy_bytes / copy_mean(cf), copy_bytes / copy_mean(cw)
     names = FP64_KERNELS
-    out = {"n": n, "kernel_sources_sha256": kernel_sources_sha(), "fetch_factor": ff, "write_factor": wf,
+    try:
+        import subprocess
+        commit = subprocess.run(["git", "-C", ROOT, "rev-parse", "--short", "HEAD"], capture_output=True,
+                                text=True, check=True).stdout.strip()
+    except (OSError, subprocess.CalledProcessError):
+        commit = None
+    out = {"n": n, "kernel_sources_sha256": kernel_sources_sha(), "kernel_sources_commit": commit,
+           "fetch_factor": ff, "write_factor": wf,
            "algorithmic_bytes_per_launch": 16 * n ** 4, "kernels": {}}
     for key, kname in names.items():
         fr = [v for k, lst in fetch.items() if k.startswith(kname + "(") or k == kname for v in lst]
