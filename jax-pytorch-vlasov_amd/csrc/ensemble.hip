@@ -274,19 +274,20 @@ __global__ __launch_bounds__(kEnsNT) void ens_setup_kernel(const vp_ens_inst* __
 // column (0 past the grid).
 __device__ __forceinline__ double x_load(const double* pool, const double* tables, const XItem& d,
                                          double (&pre)[kEnsPer]) {
+    // element e = t + k * kEnsNT: column t mod cols for every k (kEnsNT is a
+    // multiple of cols), row (t >> shift) + k * (kEnsNT >> shift)
+    const int t = static_cast<int>(threadIdx.x);
     const int shift = d.shift, cols = 1 << shift, n = d.nx << shift;
+    const int c = t & (cols - 1);
+    const bool in_grid = c < d.ncol;
+    const double* p = pool + d.f + static_cast<int64_t>(t >> shift) * d.nv + c;
+    const int64_t step = static_cast<int64_t>(kEnsNT >> shift) * d.nv;
 #pragma unroll
     for (int k = 0; k < kEnsPer; ++k) {
-        const int e = static_cast<int>(threadIdx.x) + k * kEnsNT;
-        double v = 0.0;
-        if (e < n) {
-            const int r = e >> shift, cc = e & (cols - 1);
-            if (cc < d.ncol) v = __builtin_nontemporal_load(pool + d.f + static_cast<int64_t>(r) * d.nv + cc);
-        }
-        pre[k] = v;
+        pre[k] = (t + k * kEnsNT < n && in_grid) ? __builtin_nontemporal_load(p) : 0.0;
+        p += step;
     }
-    const int c = static_cast<int>(threadIdx.x) & (cols - 1);
-    return c < d.ncol ? tables[d.v + c] : 0.0;
+    return in_grid ? tables[d.v + c] : 0.0;
 }
 
 // Window of the run of rows [r0, r0 + nr) (nr <= 8) of column c of the LDS
@@ -304,21 +305,32 @@ __device__ __forceinline__ void ens_xwin_at(double (&win)[kEnsWin], const double
     }
 }
 
-__device__ __forceinline__ void ens_xwin(const double* tile, int S, int c, int r0, int nr, double disp, int nx,
+// S: the tile's row stride, a compile-time constant, so that a window that
+// does not wrap (the usual case) reads at immediate offsets from one address.
+template <int S>
+__device__ __forceinline__ void ens_xwin(const double* tile, int c, int r0, int nr, double disp, int nx,
                                          double (&win)[kEnsWin]) {
     int j = static_cast<int>(floor(static_cast<double>(r0) - disp)) - 3;  // in [-nx, 2nx): |disp| <= nx - 2
     j += j < 0 ? nx : 0;
     j -= j >= nx ? nx : 0;
-    ens_xwin_at<0>(win, tile, S, c, nx, nr, j);
+    if (j + kEnsWin <= nx) {
+        const double* q = tile + j * S + c;
+#pragma unroll
+        for (int M = 0; M < kEnsWin; ++M) win[M] = M < nr + 7 ? q[M * S] : 0.0;
+    } else {
+        ens_xwin_at<0>(win, tile, S, c, nx, nr, j);
+    }
 }
 
-// Signed binade code of t (binade_code) without branches.
+// Signed binade code of t (binade_code) from the exponent bits: for
+// 1 <= |t| < 2^kBinades, b = floor(log2 |t|) = biased exponent - 1023 and
+// the code is b (t > 0) or -b - 1 (t < 0); otherwise (|t| < 1, zero,
+// subnormal, too large, inf, nan) kNoCode.
 __device__ __forceinline__ int ens_code(double t) {
-    int e;
-    frexp(t, &e);
-    const bool pos = t >= 1.0, neg = t <= -1.0;
-    const int c = pos ? e - 1 : -e;
-    return ((pos || neg) && e <= kBinades) ? c : kNoCode;
+    const int hi = __double2hiint(t);
+    const int b = ((hi >> 20) & 0x7ff) - 1023;
+    const bool ok = static_cast<unsigned>(b) < static_cast<unsigned>(kBinades);
+    return ok ? (hi >= 0 ? b : -b - 1) : kNoCode;
 }
 
 // Element I of a run: line index j0 + I, departure point dep, weights of its
@@ -365,25 +377,39 @@ __device__ __forceinline__ bool ens_run_moved(int j0, int nr, double disp, doubl
 // One X half-step of the run of rows [r0, r0 + nr) of column c from its
 // window, written to the tile (row stride S); tables of the column at tab
 // (codes q0 .. q0 + nq - 1, then the two special rows; stride 6).
-template <int I, bool SEL>
-__device__ __forceinline__ void ens_xrun_at(double* tile, int S, int c, int r0, int nr, double disp, double fl0, int rs,
+template <int S, int I, bool SEL>
+__device__ __forceinline__ void ens_xrun_at(double* col, int r0, int nr, double disp, double fl0, int rs,
                                             const double* tab, int q0, int nq, const double (&win)[kEnsWin]) {
     if constexpr (I < kEnsPer) {
         const double res = ens_elem<I, SEL>(win, r0, nr, disp, fl0, rs, tab, q0, nq);
-        if (I < nr) tile[(r0 + I) * S + c] = res;
-        ens_xrun_at<I + 1, SEL>(tile, S, c, r0, nr, disp, fl0, rs, tab, q0, nq, win);
+        if (I < nr) col[I * S] = res;
+        ens_xrun_at<S, I + 1, SEL>(col, r0, nr, disp, fl0, rs, tab, q0, nq, win);
     }
 }
 
-__device__ __forceinline__ void ens_xrun(double* tile, int S, int c, int r0, int nr, double disp, int rs,
-                                         const double* tab, int q0, int nq, const double (&win)[kEnsWin]) {
+template <int S>
+__device__ __forceinline__ void ens_xrun(double* tile, int c, int r0, int nr, double disp, int rs, const double* tab,
+                                         int q0, int nq, const double (&win)[kEnsWin]) {
     const double fl0 = floor(static_cast<double>(r0) - disp);
+    double* col = tile + r0 * S + c;  // row r0 of the thread's column: results at immediate offsets
     if (!ens_run_moved(r0, nr, disp, fl0)) {
-        ens_xrun_at<0, false>(tile, S, c, r0, nr, disp, fl0, rs, tab, q0, nq, win);
+        ens_xrun_at<S, 0, false>(col, r0, nr, disp, fl0, rs, tab, q0, nq, win);
     } else {
         asm volatile("");  // a branch of its own (rare)
-        ens_xrun_at<0, true>(tile, S, c, r0, nr, disp, fl0, rs, tab, q0, nq, win);
+        ens_xrun_at<S, 0, true>(col, r0, nr, disp, fl0, rs, tab, q0, nq, win);
     }
+}
+
+// The tile-pass part of an X item for a compile-time column count 1 << SH
+// (row stride S = cols + 1).
+template <int SH>
+__device__ __forceinline__ void ens_xpass_tab(double* tile, int c, int r0, int nr, double disp, int rs,
+                                              const double* tab, int q0, int nq, int nx) {
+    constexpr int S = (1 << SH) + 1;
+    double win[kEnsWin];
+    ens_xwin<S>(tile, c, r0, nr, disp, nx, win);
+    __syncthreads();  // every window has been read
+    ens_xrun<S>(tile, c, r0, nr, disp, rs, tab, q0, nq, win);
 }
 
 // Element-wise X half-step of a whole tile (items the tables do not cover):
@@ -438,12 +464,17 @@ __global__ __launch_bounds__(kEnsNT) __attribute__((amdgpu_waves_per_eu(4))) voi
         const XItem cur = nd;
         const int nx = cur.nx, shift = cur.shift, cols = 1 << shift, S = cols + 1;
         const int n = nx << shift;
-#pragma unroll
-        for (int k = 0; k < kEnsPer; ++k) {
-            const int e = t + k * kEnsNT;
-            if (e < n) tile[(e >> shift) * S + (e & (cols - 1))] = pre[k];
-        }
         const int c = t & (cols - 1);
+        {
+            // element e = t + k * kEnsNT: column c, row (t >> shift) + k * (kEnsNT >> shift)
+            double* p = tile + (t >> shift) * S + c;
+            const int step = (kEnsNT >> shift) * S;
+#pragma unroll
+            for (int k = 0; k < kEnsPer; ++k) {
+                if (t + k * kEnsNT < n) *p = pre[k];
+                p += step;
+            }
+        }
         const double disp = (vpre * half) / cur.dx;  // departure(), vlasov1D1V.py:229-238
         if (t < cols) cdisp[t] = disp;
         __syncthreads();
@@ -483,10 +514,13 @@ __global__ __launch_bounds__(kEnsNT) __attribute__((amdgpu_waves_per_eu(4))) voi
                 double dp = disp;
                 int r0p = r0;
                 asm volatile("" : "+v"(dp), "+v"(r0p));
-                double win[kEnsWin];
-                ens_xwin(tile, S, c, r0p, nr, dp, nx, win);
-                __syncthreads();  // every window has been read
-                ens_xrun(tile, S, c, r0p, nr, dp, rs, tab + c * TS, q0, nq, win);
+                const double* ct = tab + c * TS;
+                if (shift == 4)
+                    ens_xpass_tab<4>(tile, c, r0p, nr, dp, rs, ct, q0, nq, nx);
+                else if (shift == 5)
+                    ens_xpass_tab<5>(tile, c, r0p, nr, dp, rs, ct, q0, nq, nx);
+                else
+                    ens_xpass_tab<6>(tile, c, r0p, nr, dp, rs, ct, q0, nq, nx);
             } else {
                 ens_xpass_general(tile, S, nx, cols, shift, cdisp, tab);
                 __syncthreads();
@@ -494,13 +528,16 @@ __global__ __launch_bounds__(kEnsNT) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
             __syncthreads();
         }
+        if (c < cur.ncol) {
+            const double* p = tile + (t >> shift) * S + c;
+            const int step = (kEnsNT >> shift) * S;
+            double* q = pool + cur.f + static_cast<int64_t>(t >> shift) * cur.nv + c;
+            const int64_t qstep = static_cast<int64_t>(kEnsNT >> shift) * cur.nv;
 #pragma unroll
-        for (int k = 0; k < kEnsPer; ++k) {
-            const int e = t + k * kEnsNT;
-            if (e < n) {
-                const int r = e >> shift, cc = e & (cols - 1);
-                if (cc < cur.ncol)
-                    __builtin_nontemporal_store(tile[r * S + cc], pool + cur.f + static_cast<int64_t>(r) * cur.nv + cc);
+            for (int k = 0; k < kEnsPer; ++k) {
+                if (t + k * kEnsNT < n) __builtin_nontemporal_store(*p, q);
+                p += step;
+                q += qstep;
             }
         }
         if (RHO) {
