@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--summarize-counters", metavar="DIR",
                     help="write profiles/traffic_2d2v_fp64.json from the four rocprofv3 --pmc runs under DIR "
                          "and exit (no GPU needed)")
+    ap.add_argument("--only", choices=("ensemble", "surrogate"),
+                    help="run one side workload alone (for profiling) and print its record")
     ap.add_argument("--stream-only", action="store_true",
                     help="only run the HBM copy kernel on two f-sized arrays (counter calibration) and exit")
     return ap.parse_args()
@@ -646,6 +648,12 @@ def main():
     n = args.n
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble_bench(args.ensemble_steps)}), flush=True)
+        return
+    if args.only == "surrogate":
+        print(json.dumps({"surrogate": surrogate_bench(args.surrogate_iters)}), flush=True)
+        return
     copy_gbs = stream_copy_gbs(n)
     if args.stream_only:
         print(json.dumps({"stream_copy_GBps": copy_gbs, "bytes_per_launch": 2 * 8 * n ** 4}), flush=True)

@@ -407,7 +407,7 @@ class Vlasov:
     def __init__(self, Nx=32, Ny=32, Nvx=32, Nvy=32, Lx=2.0 * math.pi, Ly=2.0 * math.pi, Vx_max=6.0,
                  Vy_max=6.0, nbiter=10, diag_steps=10, dt=0.05, out_dir="data", physics_mode=False,
                  epsilon=0.001, dtype: str = "float64", solver: int = 0, device=None,
-                 fp32_storage: str = "plain", snapshot_format: str = "npz"):
+                 fp32_storage: str = "deviation", snapshot_format: str = "npz"):
         """Same parameters as the reference constructor (P2:1236-1314).
 
         ``solver`` selects the step as the reference's ``solver_fns`` does:
@@ -416,11 +416,12 @@ class Vlasov:
         dtype='float32' is the FP32-storage variant: f is float32, the grid,
         charge density, Poisson solve and fields stay float64 (the
         reference's float32 mode computes everything in float32).
-        ``fp32_storage`` (float32 only): "plain" stores f itself; "deviation"
-        stores g = f - feq in float32, feq = fx(vx) fy(vy) the separable part
-        of the (x, y) mean of the initial f with float64 profiles (used when
-        the shape has those kernels: nvy = 128), so that rounding scales with
-        the perturbation g instead of f.
+        ``fp32_storage`` (float32 only): "deviation" (the default) stores
+        g = f - feq in float32, feq = fx(vx) fy(vy) the separable part of the
+        (x, y) mean of the initial f with float64 profiles, so that rounding
+        scales with the perturbation g instead of f (used when the shape has
+        those kernels, nvy = 128; otherwise f itself is stored); "plain"
+        stores f itself.
         """
         self.output_log = ""
         if dtype not in ("float64", "float32"):
