@@ -563,8 +563,14 @@ __device__ __forceinline__ int brev_lg(int k, int lg) {
 }
 
 // rows x0 .. x0 + rows - 1: rho (from `rho` or merged partial sums), delta_rho,
-// rfft along y -> R[x][k], k <= ny/2.  The partial sums of a point are
-// loaded eight at a time and merged in part order (the order of p2d_rows_fwd).
+// rfft along y -> R[x][k], k <= ny/2.  TPP = blockDim.x / (rows * ny)
+// threads per point (a power of two, at most 8; 1 when points outnumber
+// threads): thread part h of a point loads the partial sums h, h + TPP, ...
+// all at once (up to kMergeLoads in flight), the stage twiddles are computed
+// while they land, then the thread merges them in order (compensated) and the
+// TPP threads of the point merge their sums by lane shuffles.
+constexpr int kMergeLoads = 24;
+
 __global__ __launch_bounds__(256) void p2d_fft_rows_fwd(const double* __restrict__ rho,
                                                         const double2* __restrict__ partial, int nparts, double dvx,
                                                         double dvy, int nx, int ny, int lgy, int rows,
@@ -573,31 +579,65 @@ __global__ __launch_bounds__(256) void p2d_fft_rows_fwd(const double* __restrict
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double2* tw = reinterpret_cast<double2*>(smem_raw);  // [ny - 1]
     double2* a = tw + ny;                                  // [rows][ny]
-    const int x0 = blockIdx.x * rows;
-    fill_stage_twiddles(tw, ny);
+    const int x0 = static_cast<int>(blockIdx.x) * rows;
     const int64_t npoints = static_cast<int64_t>(nx) * ny;
-    for (int idx = threadIdx.x; idx < rows * ny; idx += blockDim.x) {
-        const int64_t pt = static_cast<int64_t>(x0) * ny + idx;  // x * ny + y
-        double rv;
-        if (partial) {
-            const double2* pp = partial + pt;  // layout [part][x * ny + y]
-            KSum k{0.0, 0.0};
-            int i = 0;
-            for (; i + 8 <= nparts; i += 8) {
-                double2 v[8];
+    const int npts = rows * ny;
+    int tpp = static_cast<int>(blockDim.x) / npts;
+    tpp = tpp >= 8 ? 8 : (tpp >= 4 ? 4 : (tpp >= 2 ? 2 : 1));
+    if (partial && tpp > 1 && nparts <= tpp * kMergeLoads) {
+        // one point per TPP consecutive threads; the threads past npts * tpp idle
+        const int t = threadIdx.x;
+        const int idx = t / tpp, h = t - idx * tpp;
+        const bool active = idx < npts;
+        const int64_t pt = static_cast<int64_t>(x0) * ny + (active ? idx : 0);
+        const double2* pp = partial + pt;
+        double2 v[kMergeLoads];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = pp[(i + u) * npoints];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) ksum_merge(k, KSum{v[u].x, v[u].y});
-            }
-            for (; i < nparts; ++i) ksum_merge(k, KSum{pp[i * npoints].x, pp[i * npoints].y});
-            rv = ((k.s + k.c) * dvx) * dvy;
-        } else {
-            rv = rho[pt];
+        for (int u = 0; u < kMergeLoads; ++u) {
+            const int i = h + u * tpp;
+            v[u] = (active && i < nparts) ? pp[i * npoints] : make_double2(0.0, 0.0);
         }
-        const double d = rho_shift ? rv + rho_shift[0] : rv - 1.0;
-        a[idx] = make_double2(d, 0.0);
-        if (delta_rho) delta_rho[pt] = d;
+        fill_stage_twiddles(tw, ny);
+        KSum k{0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < kMergeLoads; ++u) ksum_merge(k, KSum{v[u].x, v[u].y});
+        for (int off = 1; off < tpp; off <<= 1) {
+            KSum o;
+            o.s = __shfl_xor(k.s, off, kWave);
+            o.c = __shfl_xor(k.c, off, kWave);
+            ksum_merge(k, o);
+        }
+        if (active && h == 0) {
+            const double rv = ((k.s + k.c) * dvx) * dvy;
+            const double d = rho_shift ? rv + rho_shift[0] : rv - 1.0;
+            a[idx] = make_double2(d, 0.0);
+            if (delta_rho) delta_rho[pt] = d;
+        }
+    } else {
+        fill_stage_twiddles(tw, ny);
+        for (int idx = threadIdx.x; idx < npts; idx += blockDim.x) {
+            const int64_t pt = static_cast<int64_t>(x0) * ny + idx;  // x * ny + y
+            double rv;
+            if (partial) {
+                const double2* pp = partial + pt;  // layout [part][x * ny + y]
+                KSum k{0.0, 0.0};
+                int i = 0;
+                for (; i + 8 <= nparts; i += 8) {
+                    double2 v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = pp[(i + u) * npoints];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) ksum_merge(k, KSum{v[u].x, v[u].y});
+                }
+                for (; i < nparts; ++i) ksum_merge(k, KSum{pp[i * npoints].x, pp[i * npoints].y});
+                rv = ((k.s + k.c) * dvx) * dvy;
+            } else {
+                rv = rho[pt];
+            }
+            const double d = rho_shift ? rv + rho_shift[0] : rv - 1.0;
+            a[idx] = make_double2(d, 0.0);
+            if (delta_rho) delta_rho[pt] = d;
+        }
     }
     __syncthreads();
     lines_forward(a, rows, ny, lgy, tw);
@@ -611,28 +651,49 @@ __global__ __launch_bounds__(256) void p2d_fft_rows_fwd(const double* __restrict
 
 // column ky_i: c2c FFT along x, phi_k = R_k inv_ksq, Ex_k = -i kx phi_k,
 // Ey_k = -i ky phi_k, three inverse FFTs along x (scaled 1/nx) -> T[q][x][ky_i].
+// blockDim.x == 256 and nx <= 512: each thread owns slots threadIdx.x and
+// threadIdx.x + 256; the column and the inv_ksq and kx values of a thread's
+// slots (bins in bit-reversed order) are loaded before the twiddles are
+// computed.
 __global__ __launch_bounds__(256) void p2d_fft_cols(const double2* __restrict__ R, int nx, int ny, int lgx,
                                                     const double* __restrict__ kx, const double* __restrict__ ky,
                                                     const double* __restrict__ inv_ksq, double2* __restrict__ T) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double2* tw = reinterpret_cast<double2*>(smem_raw);  // [nx - 1]
     double2* a = tw + nx;                                  // [4][nx]: the column, then phi_k, Ex_k, Ey_k
+    const int ky_i = static_cast<int>(blockIdx.x);
     const int nky = ny / 2 + 1;
-    const int ky_i = blockIdx.x;
+    double iks[2] = {0.0, 0.0}, kxs[2] = {0.0, 0.0};
+    double2 col[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int s = threadIdx.x + u * blockDim.x;
+        if (s < nx) {
+            const int k = brev_lg(s, lgx);  // slot s holds bin k
+            iks[u] = inv_ksq[static_cast<int64_t>(k) * nky + ky_i];
+            kxs[u] = kx[k];
+            col[u] = R[static_cast<int64_t>(s) * nky + ky_i];
+        }
+    }
+    const double kyv = ky[ky_i];
     fill_stage_twiddles(tw, nx);
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) a[x] = R[static_cast<int64_t>(x) * nky + ky_i];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int x = threadIdx.x + u * blockDim.x;
+        if (x < nx) a[x] = col[u];
+    }
     __syncthreads();
     lines_forward(a, 1, nx, lgx, tw);
-    const double kyv = ky[ky_i];
-    for (int s = threadIdx.x; s < nx; s += blockDim.x) {
-        const int k = brev_lg(s, lgx);  // slot s holds bin k
-        const double2 X = a[s];
-        const double iks = inv_ksq[static_cast<int64_t>(k) * nky + ky_i];
-        const double pr = X.x * iks, pi = X.y * iks;
-        const double kxv = kx[k];
-        a[nx + s] = make_double2(pr, pi);                  // phi_k
-        a[2 * nx + s] = make_double2(pi * kxv, -pr * kxv);  // Ex_k = -i kx phi_k
-        a[3 * nx + s] = make_double2(pi * kyv, -pr * kyv);  // Ey_k = -i ky phi_k
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int s = threadIdx.x + u * blockDim.x;
+        if (s < nx) {
+            const double2 X = a[s];
+            const double pr = X.x * iks[u], pi = X.y * iks[u];
+            a[nx + s] = make_double2(pr, pi);                        // phi_k
+            a[2 * nx + s] = make_double2(pi * kxs[u], -pr * kxs[u]);  // Ex_k = -i kx phi_k
+            a[3 * nx + s] = make_double2(pi * kyv, -pr * kyv);        // Ey_k = -i ky phi_k
+        }
     }
     __syncthreads();
     lines_inverse(a + nx, 3, nx, lgx, tw);
@@ -649,6 +710,8 @@ __global__ __launch_bounds__(256) void p2d_fft_cols(const double2* __restrict__ 
 // ny/2; the imaginary parts of the DC and Nyquist bins dropped, as irfft
 // does).  Ex and Ey are real, so one inverse FFT of the Hermitian extension
 // of Ex_k + i Ey_k gives Ex + i Ey; phi takes a transform of its own.
+// rows * (ny/2 + 1) <= 2 * blockDim.x: each thread loads at most two bins,
+// before the twiddles are computed.
 __global__ __launch_bounds__(256) void p2d_fft_rows_inv(const double2* __restrict__ T, int nx, int ny, int lgy,
                                                         int rows, double* __restrict__ phi, double* __restrict__ Ex,
                                                         double* __restrict__ Ey) {
@@ -657,25 +720,40 @@ __global__ __launch_bounds__(256) void p2d_fft_rows_inv(const double2* __restric
     double2* a = tw + ny;                                  // [rows][2][ny]: Ex + i Ey, then phi
     const int nky = ny / 2 + 1;
     const int half = ny / 2;
-    const int x0 = blockIdx.x * rows;
+    const int nb = rows * nky;
+    const int x0 = static_cast<int>(blockIdx.x) * rows;
+    double2 P[2], E[2], Y[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int idx = threadIdx.x + u * blockDim.x;
+        if (idx < nb) {
+            const int r = idx / nky;
+            const int k = idx - r * nky;
+            const int x = x0 + r;
+            P[u] = T[(static_cast<int64_t>(0) * nx + x) * nky + k];
+            E[u] = T[(static_cast<int64_t>(1) * nx + x) * nky + k];
+            Y[u] = T[(static_cast<int64_t>(2) * nx + x) * nky + k];
+        }
+    }
     fill_stage_twiddles(tw, ny);
-    for (int idx = threadIdx.x; idx < rows * nky; idx += blockDim.x) {
-        const int r = idx / nky;
-        const int k = idx - r * nky;
-        const int x = x0 + r;
-        const double2 P = T[(static_cast<int64_t>(0) * nx + x) * nky + k];
-        const double2 E = T[(static_cast<int64_t>(1) * nx + x) * nky + k];
-        const double2 Y = T[(static_cast<int64_t>(2) * nx + x) * nky + k];
-        double2* z = a + (2 * r) * ny;
-        double2* w = z + ny;
-        if (k == 0 || k == half) {
-            z[brev_lg(k, lgy)] = make_double2(E.x, Y.x);
-            w[brev_lg(k, lgy)] = make_double2(P.x, 0.0);
-        } else {
-            z[brev_lg(k, lgy)] = make_double2(E.x - Y.y, E.y + Y.x);         // E_k + i Y_k
-            z[brev_lg(ny - k, lgy)] = make_double2(E.x + Y.y, Y.x - E.y);    // conj(E_k) + i conj(Y_k)
-            w[brev_lg(k, lgy)] = P;
-            w[brev_lg(ny - k, lgy)] = make_double2(P.x, -P.y);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int idx = threadIdx.x + u * blockDim.x;
+        if (idx < nb) {
+            const int r = idx / nky;
+            const int k = idx - r * nky;
+            const double2 p = P[u], e = E[u], y = Y[u];
+            double2* z = a + (2 * r) * ny;
+            double2* w = z + ny;
+            if (k == 0 || k == half) {
+                z[brev_lg(k, lgy)] = make_double2(e.x, y.x);
+                w[brev_lg(k, lgy)] = make_double2(p.x, 0.0);
+            } else {
+                z[brev_lg(k, lgy)] = make_double2(e.x - y.y, e.y + y.x);       // E_k + i Y_k
+                z[brev_lg(ny - k, lgy)] = make_double2(e.x + y.y, y.x - e.y);  // conj(E_k) + i conj(Y_k)
+                w[brev_lg(k, lgy)] = p;
+                w[brev_lg(ny - k, lgy)] = make_double2(p.x, -p.y);
+            }
         }
     }
     __syncthreads();
@@ -800,19 +878,23 @@ int poisson2d_from(const double* rho, const double2* partial, int nparts, double
     if (pow2(nxi) && pow2(nyi) && fft2d_enabled()) {
         const int lgx = 31 - __builtin_clz(static_cast<unsigned>(nxi));
         const int lgy = 31 - __builtin_clz(static_cast<unsigned>(nyi));
-        // rows per workgroup: 256 / ny (at least 1, at most nx): nx is a multiple
-        int rows = 256 / nyi;
+        // rows per workgroup: 256 / ny (at least 1, at most nx): nx is a
+        // multiple; merging partial sums, 128 / ny (two threads per point).
+        // The inverse rows take at most 2 * 256 bins per workgroup.
+        int rows = (partial ? 128 : 256) / nyi;
         rows = rows < 1 ? 1 : (rows > nxi ? nxi : rows);
-        hipLaunchKernelGGL(p2d_fft_rows_fwd, dim3(nxi / rows), dim3(256), (ny + rows * ny) * sizeof(double2), s, rho,
-                           partial, nparts, dvx, dvy, nxi, nyi, lgy, rows, rho_shift, delta_rho, R);
-        VP_CHECK_LAUNCH();
-        hipLaunchKernelGGL(p2d_fft_cols, dim3(nky), dim3(256), 5 * nx * sizeof(double2), s, R, nxi, nyi, lgx, kx, ky,
-                           inv_ksq, T);
-        VP_CHECK_LAUNCH();
         int rows_i = 256 / nyi;
         rows_i = rows_i < 1 ? 1 : (rows_i > nxi ? nxi : rows_i);
-        hipLaunchKernelGGL(p2d_fft_rows_inv, dim3(nxi / rows_i), dim3(256), (ny + 2 * rows_i * ny) * sizeof(double2),
-                           s, T, nxi, nyi, lgy, rows_i, phi, Ex, Ey);
+        const size_t lds_f = (ny + rows * ny) * sizeof(double2);
+        const size_t lds_c = 5 * nx * sizeof(double2);
+        const size_t lds_i = (ny + 2 * rows_i * ny) * sizeof(double2);
+        hipLaunchKernelGGL(p2d_fft_rows_fwd, dim3(nxi / rows), dim3(256), lds_f, s, rho, partial, nparts, dvx, dvy,
+                           nxi, nyi, lgy, rows, rho_shift, delta_rho, R);
+        VP_CHECK_LAUNCH();
+        hipLaunchKernelGGL(p2d_fft_cols, dim3(nky), dim3(256), lds_c, s, R, nxi, nyi, lgx, kx, ky, inv_ksq, T);
+        VP_CHECK_LAUNCH();
+        hipLaunchKernelGGL(p2d_fft_rows_inv, dim3(nxi / rows_i), dim3(256), lds_i, s, T, nxi, nyi, lgy, rows_i, phi,
+                           Ex, Ey);
         VP_CHECK_LAUNCH();
         return VP_OK;
     }
