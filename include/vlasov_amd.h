@@ -147,6 +147,17 @@ int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, 
                      double dx, double dv, int physics_mode, double* E, double* rho,
                      double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
 
+/* FP32-storage form of vp_step_1d1v_f64 (no reference counterpart: the
+ * reference's float32 mode, P1:782, computes everything in float32): f in
+ * float32, the sweeps as vp_advect_f32 (float64 departure points and
+ * weights, float32 6-tap sums), rho summed in float64 from the float32
+ * values, Poisson, E, rho and phi in float64.  Workspace:
+ * vp_step_1d1v_workspace. */
+int vp_step_1d1v_f32(const float* f_in, float* f_out, int64_t nx, int64_t nv, double dt,
+                     const double* v, const double* inv_kx, const double* inv_kx_sq,
+                     double dx, double dv, int physics_mode, double* E, double* rho,
+                     double* phi, void* ws, size_t ws_bytes, vp_stream_t stream);
+
 /* vp_step_1d1v_f64 without the workspace argument and without allocating:
  * no counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

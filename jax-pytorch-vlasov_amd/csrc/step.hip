@@ -55,16 +55,27 @@ extern "C" size_t vp_step_1d1v_workspace(int64_t nx, int64_t nv) {
            vp::align256(static_cast<size_t>(nx) * sizeof(double)) + vp::poisson1d_ws(nx);
 }
 
-extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
-                                const double* v, const double* inv_kx, const double* inv_kx_sq, double dx,
-                                double dv, int physics_mode, double* E, double* rho, double* phi, void* ws,
-                                size_t ws_bytes, vp_stream_t stream) {
+namespace {
+int advect_s(const double* a, double* b, const int64_t shape[4], int axis, const double* vel, const int64_t vs[4],
+             double dt, double dx, int periodic, vp_stream_t stream) {
+    return vp_advect_f64(a, b, shape, axis, vel, vs, dt, dx, periodic, stream);
+}
+int advect_s(const float* a, float* b, const int64_t shape[4], int axis, const double* vel, const int64_t vs[4],
+             double dt, double dx, int periodic, vp_stream_t stream) {
+    return vp_advect_f32(a, b, shape, axis, vel, vs, dt, dx, periodic, stream);
+}
+
+// T = double: the reference step; T = float: FP32 storage (float64 rho, fields).
+template <typename T>
+int step_1d1v(const T* f_in, T* f_out, int64_t nx, int64_t nv, double dt, const double* v, const double* inv_kx,
+              const double* inv_kx_sq, double dx, double dv, int physics_mode, double* E, double* rho, double* phi,
+              void* ws, size_t ws_bytes, vp_stream_t stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     VP_RETURN_IF(!f_in || !f_out || !v || !E || !rho || !phi || !ws, VP_ERR_ARG);
     VP_RETURN_IF(physics_mode && (!inv_kx || !inv_kx_sq), VP_ERR_ARG);
     VP_RETURN_IF(ws_bytes < vp_step_1d1v_workspace(nx, nv), VP_ERR_WORKSPACE);
     unsigned char* wb = static_cast<unsigned char*>(ws);
-    double* tmp = reinterpret_cast<double*>(wb);
+    T* tmp = reinterpret_cast<T*>(wb);
     wb += vp::align256(static_cast<size_t>(nx) * nv * sizeof(double));
     wb += vp::align256(static_cast<size_t>(nx) * sizeof(double));
     void* pws = wb;
@@ -72,18 +83,35 @@ extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, i
     const int64_t vs_x[4] = {0, 0, 0, 1};  // drift along x with velocity v[j]
     const int64_t vs_v[4] = {0, 0, 1, 0};  // kick along v with field E[i]
     const double half = dt / 2.0;
-    VP_TRY(vp_advect_f64(f_in, f_out, shape, 2, v, vs_x, half, dx, 1, stream));
+    VP_TRY(advect_s(f_in, f_out, shape, 2, v, vs_x, half, dx, 1, stream));
     if (physics_mode) {
-        VP_TRY(vp::row_sum<double>(f_out, nx, nv, dv, 1.0, 1.0, rho, s));
+        VP_TRY(vp::row_sum<T>(f_out, nx, nv, dv, 1.0, 1.0, rho, s));
         VP_TRY(vp::poisson1d(rho, nx, inv_kx, inv_kx_sq, E, phi, pws, vp::poisson1d_ws(nx), s));
     } else {
         VP_TRY(vp::fill(E, nx, 0.0, s));
         VP_TRY(vp::fill(rho, nx, 0.0, s));
         VP_TRY(vp::fill(phi, nx, 0.0, s));
     }
-    VP_TRY(vp_advect_f64(f_out, tmp, shape, 3, E, vs_v, dt, dv, 0, stream));
-    VP_TRY(vp_advect_f64(tmp, f_out, shape, 2, v, vs_x, half, dx, 1, stream));
+    VP_TRY(advect_s(f_out, tmp, shape, 3, E, vs_v, dt, dv, 0, stream));
+    VP_TRY(advect_s(tmp, f_out, shape, 2, v, vs_x, half, dx, 1, stream));
     return VP_OK;
+}
+}  // namespace
+
+extern "C" int vp_step_1d1v_f64(const double* f_in, double* f_out, int64_t nx, int64_t nv, double dt,
+                                const double* v, const double* inv_kx, const double* inv_kx_sq, double dx,
+                                double dv, int physics_mode, double* E, double* rho, double* phi, void* ws,
+                                size_t ws_bytes, vp_stream_t stream) {
+    return step_1d1v<double>(f_in, f_out, nx, nv, dt, v, inv_kx, inv_kx_sq, dx, dv, physics_mode, E, rho, phi, ws,
+                             ws_bytes, stream);
+}
+
+extern "C" int vp_step_1d1v_f32(const float* f_in, float* f_out, int64_t nx, int64_t nv, double dt,
+                                const double* v, const double* inv_kx, const double* inv_kx_sq, double dx,
+                                double dv, int physics_mode, double* E, double* rho, double* phi, void* ws,
+                                size_t ws_bytes, vp_stream_t stream) {
+    return step_1d1v<float>(f_in, f_out, nx, nv, dt, v, inv_kx, inv_kx_sq, dx, dv, physics_mode, E, rho, phi, ws,
+                            ws_bytes, stream);
 }
 
 // ---------------------------------------------------------------------------

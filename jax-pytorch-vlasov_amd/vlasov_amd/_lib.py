@@ -42,6 +42,8 @@ _SIGS = {
     "vp_step_1d1v_workspace": (c_sz, [c_i64, c_i64]),
     "vp_step_1d1v_f64": (c_int, [c_vp, c_vp, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp, c_dbl, c_dbl,
                                  c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "vp_step_1d1v_f32": (c_int, [c_vp, c_vp, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp, c_dbl, c_dbl,
+                                 c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
     "vp_step_1d1v_f64_noalloc": (c_int, [c_vp, c_vp, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_int,
                                          c_vp, c_vp, c_vp, c_vp]),
     "vp_step_1d1v_shift_f64": (c_int, [c_vp, c_vp, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp,

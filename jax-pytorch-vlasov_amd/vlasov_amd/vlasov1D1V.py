@@ -266,9 +266,13 @@ def vlasov_step(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, x_indices, v_indices
 
 
 def step_with_field(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, grid_dx, grid_dv, physics_mode):
-    """One vlasov_step that also returns the mid-step field E: (f, rho, phi, E)."""
+    """One vlasov_step that also returns the mid-step field E: (f, rho, phi, E).
+    A float32 f runs the FP32-storage step (vp_step_1d1v_f32): f stays float32,
+    rho, phi and E are float64."""
     dev = require_gpu(f, grid_v)
-    f_c = _f64(f).contiguous()
+    if f.dtype not in (torch.float64, torch.float32):
+        raise TypeError(f"f must be float64 or float32 (FP32 storage), got {f.dtype}")
+    f_c = f.contiguous()
     Nx, Nv = f_c.shape
     out = torch.empty_like(f_c)
     E = torch.empty(Nx, dtype=torch.float64, device=dev)
@@ -277,7 +281,8 @@ def step_with_field(f, dt, grid_v, grid_inv_kx, grid_inv_kx_sq, grid_dx, grid_dv
     nbytes = lib().vp_step_1d1v_workspace(Nx, Nv)
     ws = _lib.Workspace.get(dev, nbytes, "step1d")
     pm = bool(physics_mode)
-    rc = lib().vp_step_1d1v_f64(
+    fn = lib().vp_step_1d1v_f64 if f_c.dtype == torch.float64 else lib().vp_step_1d1v_f32
+    rc = fn(
         f_c.data_ptr(), out.data_ptr(), Nx, Nv, float(dt), _f64(grid_v).contiguous().data_ptr(),
         ptr(grid_inv_kx.contiguous()) if pm else None, ptr(grid_inv_kx_sq.contiguous()) if pm else None,
         float(grid_dx), float(grid_dv), int(pm), E.data_ptr(), rho.data_ptr(), phi.data_ptr(),
@@ -456,12 +461,17 @@ class Vlasov:
 
         ``solver`` selects the step as in the reference: 0 vlasov_step,
         1 vlasov_step_matrix_diag, 2 vlasov_step_gather_free (static shift
-        data hoisted out of the loop).  Only float64 is supported by this
-        class.
+        data hoisted out of the loop).  dtype='float32' is the FP32-storage
+        variant (solver 0): f is float32, the grid, rho, phi and E stay
+        float64 (the reference's float32 mode computes everything in
+        float32).
         """
         self.output_log = ""
-        if dtype != "float64":
-            raise ValueError("vlasov_amd.vlasov1D1V.Vlasov runs float64 (dtype='float64')")
+        if dtype not in ("float64", "float32"):
+            raise ValueError(f"dtype must be 'float64' or 'float32', got {dtype!r}")
+        if dtype == "float32" and solver != 0:
+            raise ValueError("solvers 1 and 2 run float64 (the FP32-storage variant uses solver 0)")
+        self.f_dtype = torch.float64 if dtype == "float64" else torch.float32
         self.dtype = torch.float64
         self.device = torch.device(device) if device is not None else default_device()
         self.grid = Grid.create(Nx, Nv, L, V_max, device=self.device, dtype=self.dtype)
@@ -497,7 +507,7 @@ class Vlasov:
         pathlib.Path(self.out_dir).mkdir(parents=True, exist_ok=True)
         num_steps = self.nbiter // self.diag_steps
         message = f"PyTorch version: {torch.__version__}\n"
-        message += f"Starting simulation on {self.device} with {self.grid.x.dtype} precision "
+        message += f"Starting simulation on {self.device} with {self.f_dtype} precision "
         message += "with Poisson solver" if self.physics_mode else "without Poisson solver"
         message += f" and solver type {self.solver_type}."
         self._append_log(message)
@@ -522,6 +532,7 @@ class Vlasov:
             sigma_x, sigma_v = self.grid.L / 10.0, self.grid.V_max / 4.0
             f = torch.exp(-0.5 * ((x_2d - x0) ** 2 / sigma_x ** 2 + (v_2d - v0) ** 2 / sigma_v ** 2))
             f0 = torch.zeros_like(self.grid.v, dtype=self.dtype)
+        f = f.to(self.f_dtype).contiguous()
         E, rho, phi = solve_poisson_real(f, self.grid.inv_kx, self.grid.inv_kx_sq, self.grid.dv)
         state = SimulationState(field=Field(rho, phi), variables=Variables(fn=f, f0=f0), t=0.0, step=0)
         self.save_diag(state)
@@ -549,12 +560,13 @@ class Vlasov:
     def restart(self, step: int) -> SimulationState:
         """State of the raw binary snapshot of iteration ``step`` in out_dir
         (written with snapshot_format='bin', or by the reference's Kokkos
-        solver); run() continues from it."""
+        solver); run() continues from it.  f takes this run's precision."""
         from . import snapshots
         g = self.grid
         a = snapshots.read_snapshot(self.out_dir, step, {"fn": (g.Nx, g.Nv), "f0": (g.Nv,), "rho": (g.Nx,),
                                                          "phi": (g.Nx,)})
         t = {k: torch.from_numpy(v.astype(np.float64)).to(self.device) for k, v in a.items()}
+        t["fn"] = torch.from_numpy(a["fn"]).to(self.device).to(self.f_dtype)
         t["rho"] = snapshots.delta_rho_of(t["rho"])  # the files hold the full density
         return SimulationState(field=Field(t["rho"], t["phi"]), variables=Variables(fn=t["fn"], f0=t["f0"]),
                                t=float(step) * self.dt, step=int(step))
