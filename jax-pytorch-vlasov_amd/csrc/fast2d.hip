@@ -399,55 +399,6 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     const int64_t base = q * KC + c;  // + a * a_stride of the unit
     const int j = LANE_W ? (PAIRW ? 2 * (q * KC + c) : q * KC + c) : u_begin / nsplit;
 
-    S w[R][6];  // weights (of the first column when PAIRW, else shared by a pair)
-    uint32_t A[R];
-    float w1[PAIRW ? R : 1][6];   // PAIRW: the second column's weights,
-    uint32_t A1[PAIRW ? R : 1];   // the byte offset of its tap-0 row,
-    bool split[PAIRW ? R : 1];    // and whether that row differs from the first column's
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        const int r = g + KG * i;
-        double wd[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) wd[k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
-        round_weights<S>(wd, w[i]);
-        const int il = IL[static_cast<int64_t>(r) * nvel + j];
-        A[i] = static_cast<uint32_t>(((il - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(E)));
-        if constexpr (PAIRW) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) wd[k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j + 1];
-            round_weights<float>(wd, w1[i]);
-            const int il1 = IL[static_cast<int64_t>(r) * nvel + j + 1];
-            A1[i] = static_cast<uint32_t>(((il1 - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(E)) + 4);
-            split[i] = il1 != il;
-        }
-    }
-    // Consume the weight loads here: otherwise the wait-count analysis of the
-    // loop body also waits for the in-flight stores before each FMA.
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w[i][k]));
-        asm volatile("" ::"v"(A[i]));
-        if constexpr (PAIRW) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w1[i][k]));
-            asm volatile("" ::"v"(A1[i]));
-        }
-    }
-    bool wsplit[PAIRW ? R : 1];  // some lane of the wave reads the row column by column
-#pragma unroll
-    for (int i = 0; i < (PAIRW ? R : 1); ++i) wsplit[i] = PAIRW && __any(split[i]);
-    // float32 storage: rows whose taps wrap for no lane of the wave (all but
-    // a few next to the line ends) read them at immediate offsets, dropping
-    // two integer operations per tap; float32 interpolation costs as many
-    // VALU cycles per element as float64 (no packed math), so the float
-    // drifts are the ones this saves time in (2 %; float64: no change).
-    constexpr bool DIRECT = sizeof(S) == 4;
-    bool direct[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) direct[i] = DIRECT && __all((A[i] >> 8) <= static_cast<uint32_t>(n - 6));
-
     // Walk over the units' tiles (wave-uniform): tile k of unit u is loop
     // index l0 + (r0 + k) mod len.
     struct Walk {
@@ -499,6 +450,58 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
     };
 #pragma unroll
     for (int u = 0; u < T - 1; ++u) issue(u);
+
+    // The weights load while the first tiles are in flight (their loads were
+    // issued first: a workgroup lives for a few tiles, and the table reads
+    // would otherwise delay its first tile by an L2 round trip).
+    S w[R][6];  // weights (of the first column when PAIRW, else shared by a pair)
+    uint32_t A[R];
+    float w1[PAIRW ? R : 1][6];   // PAIRW: the second column's weights,
+    uint32_t A1[PAIRW ? R : 1];   // the byte offset of its tap-0 row,
+    bool split[PAIRW ? R : 1];    // and whether that row differs from the first column's
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = g + KG * i;
+        double wd[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wd[k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j];
+        round_weights<S>(wd, w[i]);
+        const int il = IL[static_cast<int64_t>(r) * nvel + j];
+        A[i] = static_cast<uint32_t>(((il - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(E)));
+        if constexpr (PAIRW) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wd[k] = W[(static_cast<int64_t>(r) * 6 + k) * nvel + j + 1];
+            round_weights<float>(wd, w1[i]);
+            const int il1 = IL[static_cast<int64_t>(r) * nvel + j + 1];
+            A1[i] = static_cast<uint32_t>(((il1 - 2) & (n - 1)) * 256 + c * static_cast<int>(sizeof(E)) + 4);
+            split[i] = il1 != il;
+        }
+    }
+    // Consume the weight loads here: otherwise the wait-count analysis of the
+    // loop body also waits for the in-flight stores before each FMA.
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w[i][k]));
+        asm volatile("" ::"v"(A[i]));
+        if constexpr (PAIRW) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(w1[i][k]));
+            asm volatile("" ::"v"(A1[i]));
+        }
+    }
+    bool wsplit[PAIRW ? R : 1];  // some lane of the wave reads the row column by column
+#pragma unroll
+    for (int i = 0; i < (PAIRW ? R : 1); ++i) wsplit[i] = PAIRW && __any(split[i]);
+    // float32 storage: rows whose taps wrap for no lane of the wave (all but
+    // a few next to the line ends) read them at immediate offsets, dropping
+    // two integer operations per tap; float32 interpolation costs as many
+    // VALU cycles per element as float64 (no packed math), so the float
+    // drifts are the ones this saves time in (2 %; float64: no change).
+    constexpr bool DIRECT = sizeof(S) == 4;
+    bool direct[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) direct[i] = DIRECT && __all((A[i] >> 8) <= static_cast<uint32_t>(n - 6));
 
     // Charge density of this thread's rows: compensated for float64 storage;
     // for float32 storage the float64 sum of at most 128 float32 values per
