@@ -542,8 +542,9 @@ __global__ __launch_bounds__(kEnsNT) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         if (RHO) {
             // row sums over the item's columns: tpr consecutive threads per row,
-            // cols / tpr columns each, merged by lane shuffles (columns past nv
-            // hold exact zeros: their taps read zeros)
+            // cols / tpr columns each, merged by DPP moves (columns past nv
+            // hold exact zeros: their taps read zeros); the group's last lane
+            // holds the row's sum
             int tpr = 1;
             while (tpr < 64 && tpr < cols && 2 * tpr * nx <= kEnsNT) tpr *= 2;
             const int row = t / tpr, part = t - row * tpr;
@@ -551,13 +552,8 @@ __global__ __launch_bounds__(kEnsNT) __attribute__((amdgpu_waves_per_eu(4))) voi
             KSum k{0.0, 0.0};
             if (row < nx)
                 for (int u = 0; u < cpt; ++u) ksum_add(k, tile[row * S + part * cpt + u]);
-            for (int off = 1; off < tpr; off <<= 1) {
-                KSum o2;
-                o2.s = __shfl_xor(k.s, off, 64);
-                o2.c = __shfl_xor(k.c, off, 64);
-                ksum_merge(k, o2);
-            }
-            if (row < nx && part == 0) partial[static_cast<int64_t>(g) * kEnsMaxNx + row] = make_double2(k.s, k.c);
+            k = group_ksum<true>(k, tpr);
+            if (row < nx && part == tpr - 1) partial[static_cast<int64_t>(g) * kEnsMaxNx + row] = make_double2(k.s, k.c);
         }
         __syncthreads();  // the tile, tables and s_next are free for the next item
         g = gn;

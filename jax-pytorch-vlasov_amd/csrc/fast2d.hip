@@ -147,14 +147,8 @@ __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
         const int nparts = nq * S;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-            KSum k = acc[i];
-            for (int off = kCol / 2; off > 0; off >>= 1) {
-                KSum o;
-                o.s = __shfl_xor(k.s, off, kCol);
-                o.c = __shfl_xor(k.c, off, kCol);
-                ksum_merge(k, o);
-            }
-            if (c == 0) {
+            const KSum k = group_ksum<true>(acc[i], kCol);
+            if (c == kCol - 1) {
                 const int r = g + kGroups * i;
                 // part-major layout [part][x * ny + y]: the Poisson kernel reads it coalesced
                 const int64_t npoints = static_cast<int64_t>(gridDim.x / nparts) * n;
@@ -511,26 +505,18 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
 #pragma unroll
     for (int i = 0; i < R; ++i) acc[i] = KSum{0.0, 0.0};
     // End of a unit: the row sums of its tiles over the chunk's KC columns
-    // (the lanes of one row group: a butterfly of lane shuffles), written by
-    // the group's column-0 lane.  Its stores join the vmcnt queue behind the
+    // (the lanes of one row group, merged by DPP moves in registers: a
+    // butterfly of LDS lane shuffles here was exposed at the end of every
+    // short-lived workgroup, 818-824 -> 802-806 us per 128^4 Y drift + rho),
+    // written by the group's last lane.  Its stores join the vmcnt queue behind the
     // tile stores; the waits below then wait for a little more than they
     // need to, never less.
     auto flush = [&](int u) {
         const int64_t at = static_cast<int64_t>(q * nsplit + u % nsplit) * na * n + static_cast<int64_t>(u / nsplit) * n;
 #pragma unroll
         for (int ii = 0; ii < R; ++ii) {
-            KSum k = acc[ii];
-            for (int off = KC / 2; off > 0; off >>= 1) {
-                KSum o;
-                o.s = __shfl_xor(k.s, off, KC);
-                if (COMP) {
-                    o.c = __shfl_xor(k.c, off, KC);
-                    ksum_merge(k, o);
-                } else {
-                    k.s += o.s;
-                }
-            }
-            if (c == 0) partial[at + g + KG * ii] = make_double2(k.s, COMP ? k.c : 0.0);
+            const KSum k = group_ksum<COMP>(acc[ii], KC);
+            if (c == KC - 1) partial[at + g + KG * ii] = make_double2(k.s, COMP ? k.c : 0.0);
             acc[ii] = KSum{0.0, 0.0};
         }
     };

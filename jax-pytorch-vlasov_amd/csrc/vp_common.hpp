@@ -165,6 +165,44 @@ __device__ __forceinline__ KSum wave_ksum(KSum k) {
     return k;
 }
 
+// A float64 moved across lanes by DPP (both 32-bit halves), a register move
+// with no LDS round trip; lanes the row mask leaves out keep `old`.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double old, double x) {
+    const long long xb = __double_as_longlong(x), ob = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(ob), static_cast<int>(xb), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(ob >> 32), static_cast<int>(xb >> 32), CTRL, ROW_MASK,
+                                               0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Sum over each aligned group of `width` (1, 2, 4, ..., 64; wave-uniform)
+// consecutive lanes by DPP moves, in registers: pairs (quad permutation),
+// pairs of pairs, 8 lanes (half-row mirror), 16 lanes (row mirror); every
+// lane of a group of up to 16 ends with the group's sum.  Groups of 32 add
+// row 0 into row 1 and row 2 into row 3 (row_bcast:15), groups of 64 then
+// rows 0-1 into rows 2-3 (row_bcast:31): there the group's last lane holds
+// the sum.  Merges are compensated when COMP.
+template <bool COMP>
+__device__ __forceinline__ KSum group_ksum(KSum k, int width) {
+    auto step = [&](double os, double oc) {
+        if (COMP) {
+            ksum_merge(k, KSum{os, oc});
+        } else {
+            k.s += os;
+        }
+    };
+    if (width >= 2) step(dpp_f64<0xB1>(0.0, k.s), COMP ? dpp_f64<0xB1>(0.0, k.c) : 0.0);    // quad_perm [1, 0, 3, 2]
+    if (width >= 4) step(dpp_f64<0x4E>(0.0, k.s), COMP ? dpp_f64<0x4E>(0.0, k.c) : 0.0);    // quad_perm [2, 3, 0, 1]
+    if (width >= 8) step(dpp_f64<0x141>(0.0, k.s), COMP ? dpp_f64<0x141>(0.0, k.c) : 0.0);  // row_half_mirror
+    if (width >= 16) step(dpp_f64<0x140>(0.0, k.s), COMP ? dpp_f64<0x140>(0.0, k.c) : 0.0);  // row_mirror
+    if (width >= 32)
+        step(dpp_f64<0x142, 0xA>(0.0, k.s), COMP ? dpp_f64<0x142, 0xA>(0.0, k.c) : 0.0);  // row_bcast:15 into rows 1, 3
+    if (width >= 64)
+        step(dpp_f64<0x143, 0xC>(0.0, k.s), COMP ? dpp_f64<0x143, 0xC>(0.0, k.c) : 0.0);  // row_bcast:31 into rows 2, 3
+    return k;
+}
+
 // Block-wide compensated sum; every thread gets the result.  `scratch` must
 // hold 2 * (blockDim.x / 64) doubles.
 __device__ __forceinline__ double block_ksum(KSum k, double* scratch) {
