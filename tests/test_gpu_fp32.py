@@ -198,6 +198,49 @@ def test_fp32_xdrift_column_pairs_equal_single_columns(device, shape, passes):
     assert not torch.equal(outs[0], f)
 
 
+@pytest.mark.parametrize("shape", [(64, 128, 32, 128), (64, 64, 32, 64)])
+def test_fp32_ydrift_rho_single_columns_and_pairs(device, shape):
+    """The float32 Y drift + rho with one vy column per thread (row sums over
+    groups of 64 lanes) and with pairs of columns (groups of 32): the same f
+    bits, and delta_rho of both within 1e-13 of max rho of an 80-bit sum of
+    the drifted f (a lane group merged wrongly would be off by whole partial
+    sums)."""
+    v2, runner, L = _mods()
+    G, f0 = _setup(shape, 7)
+    f = torch.from_numpy(f0).float().to(device)
+    sh = L.i64x4(shape)
+    lib = L.lib()
+    s = torch.cuda.current_stream().cuda_stream
+    nx, ny = shape[0], shape[1]
+    kx, ky, iks = (t.to(torch.float64).contiguous() for t in (G.kx, G.ky, G.inv_ksq))
+    outs = []
+    old = lib.vp_tune_get(L.KNOBS["f32_pairs"])
+    try:
+        for mode in (2, 1):
+            lib.vp_tune_set(L.KNOBS["f32_pairs"], mode)
+            ws = torch.empty(lib.vp_fused_workspace(sh, 4), dtype=torch.uint8, device=device)
+            L.check(lib.vp_fused_tables_f32(sh, G.vx.data_ptr(), G.vy.data_ptr(), 0.05, G.dx, G.dy, ws.data_ptr(),
+                                            ws.numel(), s), "tables")
+            o = torch.empty_like(f)
+            L.check(lib.vp_fused_ydrift_f32(f.data_ptr(), o.data_ptr(), sh, 1, ws.data_ptr(), ws.numel(), s),
+                    "ydrift")
+            fields = torch.empty((4, nx, ny), dtype=torch.float64, device=device)
+            L.check(lib.vp_fused_poisson_f32(sh, G.dvx, G.dvy, kx.data_ptr(), ky.data_ptr(), iks.data_ptr(),
+                                             fields[0].data_ptr(), fields[1].data_ptr(), fields[2].data_ptr(),
+                                             fields[3].data_ptr(), ws.data_ptr(), ws.numel(), s), "poisson")
+            torch.cuda.synchronize()
+            outs.append((o, fields[2].clone()))
+    finally:
+        lib.vp_tune_set(L.KNOBS["f32_pairs"], old)
+    assert torch.equal(outs[0][0], outs[1][0])
+    fh = outs[0][0].double().cpu().numpy().reshape(nx * ny, -1)
+    S = fh.astype(np.longdouble).sum(axis=1)
+    ref = (S * np.longdouble(G.dvx) * np.longdouble(G.dvy) - 1).astype(np.float64).reshape(nx, ny)
+    for mode, (_, drho) in zip((2, 1), outs):
+        err = np.max(np.abs(drho.cpu().numpy() - ref))
+        assert err <= 1e-13 * np.max(np.abs(ref + 1.0)), (mode, err)
+
+
 @pytest.mark.parametrize("shape,axis,periodic", [((1, 1, 1000, 300), 2, True), ((2, 1, 40, 2000), 3, False),
                                                  ((600, 2, 3, 70), 0, True)])
 def test_fp32_long_line_sweeps(device, shape, axis, periodic):
