@@ -145,16 +145,13 @@ __global__ __launch_bounds__(kDriftThreads) void drift_kernel(
     }
     if (RHO) {
         const int nparts = nq * S;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            const KSum k = group_ksum<true>(acc[i], kCol);
-            if (c == kCol - 1) {
-                const int r = g + kGroups * i;
-                // part-major layout [part][x * ny + y]: the Poisson kernel reads it coalesced
-                const int64_t npoints = static_cast<int64_t>(gridDim.x / nparts) * n;
-                partial[static_cast<int64_t>(q * S + s) * npoints + static_cast<int64_t>(a) * n + r] =
-                    make_double2(k.s, k.c);
-            }
+        // part-major layout [part][x * ny + y]: the Poisson kernel reads it coalesced
+        const int64_t npoints = static_cast<int64_t>(gridDim.x / nparts) * n;
+        const KSum k = rows_ksum<true, R, kCol>(acc, c);
+        if (c < R) {
+            const int r = g + kGroups * rows_ksum_row<R>(c);
+            partial[static_cast<int64_t>(q * S + s) * npoints + static_cast<int64_t>(a) * n + r] =
+                make_double2(k.s, k.c);
         }
     }
 }
@@ -505,20 +502,28 @@ __global__ __launch_bounds__(kDriftThreads) void drift_dma_kernel(
 #pragma unroll
     for (int i = 0; i < R; ++i) acc[i] = KSum{0.0, 0.0};
     // End of a unit: the row sums of its tiles over the chunk's KC columns
-    // (the lanes of one row group, merged by DPP moves in registers: a
-    // butterfly of LDS lane shuffles here was exposed at the end of every
-    // short-lived workgroup, 818-824 -> 802-806 us per 128^4 Y drift + rho),
-    // written by the group's last lane.  Its stores join the vmcnt queue behind the
+    // (the lanes of one row group), merged in registers, halving the rows a
+    // lane holds at each of the first exchanges (rows_ksum): this merge runs
+    // after the unit's last tile with no memory work to hide it, at the end of
+    // every short-lived workgroup (a butterfly of LDS lane shuffles over all R
+    // rows: 818-824 us per 128^4 Y drift + rho; by DPP over all R rows:
+    // 802-806), and is written by lanes 0 .. R-1 of the group.  Its stores join the vmcnt queue behind the
     // tile stores; the waits below then wait for a little more than they
     // need to, never less.
     auto flush = [&](int u) {
         const int64_t at = static_cast<int64_t>(q * nsplit + u % nsplit) * na * n + static_cast<int64_t>(u / nsplit) * n;
+        if constexpr (R <= 4) {
+            const KSum k = rows_ksum<COMP, R, KC>(acc, c);
+            if (c < R) partial[at + g + KG * rows_ksum_row<R>(c)] = make_double2(k.s, COMP ? k.c : 0.0);
+        } else {
 #pragma unroll
-        for (int ii = 0; ii < R; ++ii) {
-            const KSum k = group_ksum<COMP>(acc[ii], KC);
-            if (c == KC - 1) partial[at + g + KG * ii] = make_double2(k.s, COMP ? k.c : 0.0);
-            acc[ii] = KSum{0.0, 0.0};
+            for (int ii = 0; ii < R; ++ii) {
+                const KSum k = group_ksum<COMP>(acc[ii], KC);
+                if (c == KC - 1) partial[at + g + KG * ii] = make_double2(k.s, COMP ? k.c : 0.0);
+            }
         }
+#pragma unroll
+        for (int ii = 0; ii < R; ++ii) acc[ii] = KSum{0.0, 0.0};
     };
     Walk ow;  // the tile computed
     start(ow, u_begin);

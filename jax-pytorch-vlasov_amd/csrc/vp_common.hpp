@@ -203,6 +203,69 @@ __device__ __forceinline__ KSum group_ksum(KSum k, int width) {
     return k;
 }
 
+// Row sums of the R rows every lane holds (acc[R], R = 1, 2 or 4) over aligned
+// groups of W = 32 or 64 lanes, halving the rows a lane holds while reducing:
+// at the exchanges with the lane one and two apart (quad permutations) the
+// lane keeps half of its rows (the lower half where the exchange's lane bit
+// is clear) and merges the partner's copy of that half, so each step merges
+// at most R / 2 rows instead of R.  Row 2 b0 + b1 (R = 4; row b0 for R = 2)
+// ends in the lanes whose low bits are b0, b1; those lanes then merge over
+// row rotations by 4 and 8 (which keep the lane index mod 4) and across 16
+// (and 32) lanes.  The result: lane j < R of each group holds the sum of row
+// rows_ksum_row<R>(j).  Compensated merges when COMP.
+template <int R>
+__device__ __forceinline__ int rows_ksum_row(int j) {
+    return R == 4 ? 2 * (j & 1) + ((j >> 1) & 1) : (R == 2 ? (j & 1) : 0);
+}
+
+template <bool COMP>
+__device__ __forceinline__ void ksum_merge_c(KSum& a, const KSum& b) {
+    if (COMP) {
+        ksum_merge(a, b);
+    } else {
+        a.s += b.s;
+    }
+}
+
+template <int CTRL, bool COMP>
+__device__ __forceinline__ KSum dpp_ksum(const KSum& k) {
+    return KSum{dpp_f64<CTRL>(0.0, k.s), COMP ? dpp_f64<CTRL>(0.0, k.c) : 0.0};
+}
+
+// One halving exchange: keep rows [0, M/2) (upper == false) or [M/2, M) and
+// merge the partner's copy of them; the kept rows move to r[0 .. M/2).
+template <int CTRL, bool COMP, int M>
+__device__ __forceinline__ void rows_split(KSum* r, bool upper) {
+    constexpr int H = M / 2;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const KSum keep = upper ? r[H + i] : r[i];
+        const KSum send = upper ? r[i] : r[H + i];
+        KSum k = keep;
+        ksum_merge_c<COMP>(k, dpp_ksum<CTRL, COMP>(send));
+        r[i] = k;
+    }
+}
+
+template <bool COMP, int R, int W>
+__device__ __forceinline__ KSum rows_ksum(KSum (&acc)[R], int lane) {
+    static_assert(R == 1 || R == 2 || R == 4, "1, 2 or 4 rows per lane");
+    static_assert(W == 32 || W == 64, "groups of 32 or 64 lanes");
+    KSum r[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) r[i] = acc[i];
+    if constexpr (R >= 2) rows_split<0xB1, COMP, R>(r, (lane & 1) != 0);  // partner: lane ^ 1
+    if constexpr (R == 4) rows_split<0x4E, COMP, 2>(r, (lane & 2) != 0);  // partner: lane ^ 2
+    KSum k = r[0];
+    if constexpr (R == 1) ksum_merge_c<COMP>(k, dpp_ksum<0xB1, COMP>(k));  // lane ^ 1
+    if constexpr (R <= 2) ksum_merge_c<COMP>(k, dpp_ksum<0x4E, COMP>(k));  // lane ^ 2
+    ksum_merge_c<COMP>(k, dpp_ksum<0x124, COMP>(k));                      // row_ror:4
+    ksum_merge_c<COMP>(k, dpp_ksum<0x128, COMP>(k));                      // row_ror:8
+    ksum_merge_c<COMP>(k, KSum{__shfl_xor(k.s, 16, W), COMP ? __shfl_xor(k.c, 16, W) : 0.0});
+    if constexpr (W == 64) ksum_merge_c<COMP>(k, KSum{__shfl_xor(k.s, 32, 64), COMP ? __shfl_xor(k.c, 32, 64) : 0.0});
+    return k;
+}
+
 // Block-wide compensated sum; every thread gets the result.  `scratch` must
 // hold 2 * (blockDim.x / 64) doubles.
 __device__ __forceinline__ double block_ksum(KSum k, double* scratch) {
