@@ -3,6 +3,8 @@ Vlasov driver and the reference-style command line.
 
 * moments (mass, kinetic energy, L2) of f against the float64 oracle
   (compensated sums: <= 1e-13 relative), float64 and float32 storage;
+* the field energy (1/2 sum (Ex^2 + Ey^2)) dx dy at sizes that are not
+  multiples of the kernel's load batches, against exactly rounded sums;
 * the 2D2V driver with dtype='float32' tracks the float64 driver's
   field-energy history to 1e-5 and conserves mass;
 * the command line writes the log lines the reference's parsers read
@@ -18,6 +20,29 @@ import torch
 from oracle import vlasov_oracle as vo
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 100, 1023, 16384, 100003])
+def test_field_energy_exactly_rounded_sums(device, n):
+    """Compensated sums of E^2 (1D, Ey null) and Ex^2 + Ey^2 (2D): within 2
+    ulps of math.fsum of the same float64 squares and sums, then scaled as
+    the kernel scales them."""
+    from vlasov_amd import _lib
+    lib = _lib.lib()
+    rng = np.random.default_rng(n)
+    ex = rng.standard_normal(n) * 10.0 ** rng.uniform(-6, 2, n)
+    ey = rng.standard_normal(n)
+    dx, dy = 0.37, 1.9
+    tx, ty = torch.from_numpy(ex).to(device), torch.from_numpy(ey).to(device)
+    out = torch.empty(2, dtype=torch.float64, device=device)
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.vp_field_energy_f64(tx.data_ptr(), None, n, dx, 1.0, out.data_ptr(), s), "energy")
+    _lib.check(lib.vp_field_energy_f64(tx.data_ptr(), ty.data_ptr(), n, dx, dy, out.data_ptr() + 8, s), "energy")
+    got = out.cpu().numpy()
+    ref1 = ((0.5 * math.fsum(ex * ex)) * dx) * 1.0
+    ref2 = ((0.5 * math.fsum(ex * ex + ey * ey)) * dx) * dy
+    for g_, r_ in ((got[0], ref1), (got[1], ref2)):
+        assert abs(g_ - r_) <= 2 * np.finfo(np.float64).eps * abs(r_), (n, g_, r_)
 
 
 def test_moments_1d_match_oracle(device):
