@@ -211,7 +211,7 @@ __device__ __forceinline__ KSum group_ksum(KSum k, int width) {
 // at most R / 2 rows instead of R.  Row 2 b0 + b1 (R = 4; row b0 for R = 2)
 // ends in the lanes whose low bits are b0, b1; those lanes then merge over
 // row rotations by 4 and 8 (which keep the lane index mod 4) and across 16
-// (and 32) lanes.  The result: lane j < R of each group holds the sum of row
+// (and 32) lanes by cross-row swaps.  The result: lane j < R of each group holds the sum of row
 // rows_ksum_row<R>(j).  Compensated merges when COMP.
 template <int R>
 __device__ __forceinline__ int rows_ksum_row(int j) {
@@ -247,6 +247,31 @@ __device__ __forceinline__ void rows_split(KSum* r, bool upper) {
     }
 }
 
+// The value of lane ^ 16 (ROWS16) or lane ^ 32, by the gfx950 cross-row
+// swaps (VALU, no LDS round trip).  With x as both operands,
+// v_permlane16_swap returns every 16-lane row filled with the even rows' values
+// (first result) and with the odd rows' (second); v_permlane32_swap likewise
+// with the lower and upper halves of the wave.  A lane takes the other kind.
+template <bool ROWS16>
+__device__ __forceinline__ double lane_swap_f64(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = static_cast<unsigned>(b), hi = static_cast<unsigned>(b >> 32);
+    const bool upper = ROWS16 ? (lane & 16) != 0 : (lane & 32) != 0;
+    unsigned plo, phi;
+    if constexpr (ROWS16) {
+        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        plo = upper ? l[0] : l[1];
+        phi = upper ? h[0] : h[1];
+    } else {
+        const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        plo = upper ? l[0] : l[1];
+        phi = upper ? h[0] : h[1];
+    }
+    return __longlong_as_double((static_cast<long long>(phi) << 32) | plo);
+}
+
 template <bool COMP, int R, int W>
 __device__ __forceinline__ KSum rows_ksum(KSum (&acc)[R], int lane) {
     static_assert(R == 1 || R == 2 || R == 4, "1, 2 or 4 rows per lane");
@@ -261,8 +286,9 @@ __device__ __forceinline__ KSum rows_ksum(KSum (&acc)[R], int lane) {
     if constexpr (R <= 2) ksum_merge_c<COMP>(k, dpp_ksum<0x4E, COMP>(k));  // lane ^ 2
     ksum_merge_c<COMP>(k, dpp_ksum<0x124, COMP>(k));                      // row_ror:4
     ksum_merge_c<COMP>(k, dpp_ksum<0x128, COMP>(k));                      // row_ror:8
-    ksum_merge_c<COMP>(k, KSum{__shfl_xor(k.s, 16, W), COMP ? __shfl_xor(k.c, 16, W) : 0.0});
-    if constexpr (W == 64) ksum_merge_c<COMP>(k, KSum{__shfl_xor(k.s, 32, 64), COMP ? __shfl_xor(k.c, 32, 64) : 0.0});
+    ksum_merge_c<COMP>(k, KSum{lane_swap_f64<true>(k.s, lane), COMP ? lane_swap_f64<true>(k.c, lane) : 0.0});
+    if constexpr (W == 64)
+        ksum_merge_c<COMP>(k, KSum{lane_swap_f64<false>(k.s, lane), COMP ? lane_swap_f64<false>(k.c, lane) : 0.0});
     return k;
 }
 
