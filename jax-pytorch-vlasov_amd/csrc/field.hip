@@ -923,7 +923,25 @@ __global__ __launch_bounds__(1024) void energy_kernel(const double* __restrict__
                                                       int64_t n, double s1, double s2, double* __restrict__ out) {
     __shared__ double scratch[2 * 1024 / kWave];
     KSum k{0.0, 0.0};
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    constexpr int kU = 8;  // loads in flight per thread; each thread adds in index order
+    const int64_t st = blockDim.x;
+    int64_t i = threadIdx.x;
+    for (; i + (kU - 1) * st < n; i += kU * st) {
+        double av[kU], bv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) av[u] = a[i + u * st];
+        if (b) {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) bv[u] = b[i + u * st];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            double e = av[u] * av[u];
+            if (b) e = e + bv[u] * bv[u];
+            ksum_add(k, e);
+        }
+    }
+    for (; i < n; i += st) {
         const double av = a[i];
         double e = av * av;
         if (b) {
