@@ -35,7 +35,9 @@
 // integer), so floor(t_i) - floor(t_0) - i is -1, 0 or 1.  Every thread
 // first reads the window of run + 7 values its stencils can reach into
 // registers; after a barrier it computes its run from them and writes the
-// results in place.  The weights are read only when the binade of t changes.
+// results in place.  Each element reads its six weights from the column's
+// LDS table slot of its binade code (three 16-byte reads): keeping one set
+// per run and switching only where the code changes made waves diverge.
 #include "binade.hpp"
 #include "field_dev.hpp"
 
