@@ -851,6 +851,10 @@ __global__ __launch_bounds__(128) void kick_stream_kernel(const S* __restrict__ 
 // plain kernel, plus a float64 correction that needs per plane the 1D kicks
 // of the two profiles (ax and fx of every row in LDS, ay and fy of the
 // lane's columns in registers) and nothing from memory per element.
+// Rows of a plane whose departure offset may differ from the middle row's
+// for the windowed kick walks (their taps are loaded directly).
+constexpr int kKickMaxDev = 4;
+
 template <typename S, int NVX, int WPP, int PF, bool NT, bool DEV, int NVY = 128>
 __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict__ fin, S* __restrict__ fout,
                                                              const double* __restrict__ feq,
@@ -873,7 +877,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     typedef typename std::conditional<CPL == 2, V2_, C>::type V2;    // and in arithmetic
     __shared__ C wx[NVX * 6];
     __shared__ int ilx[NVX];
-    __shared__ int irregular[1];
+    __shared__ int ndev[1];
     __shared__ C rowbuf[WPP][2][NVY + 2 * PAD];
     __shared__ double axs[DEV ? NVX : 1], fxs[DEV ? NVX : 1];  // DEV: Kvx fx and fx of every row
     const int t = threadIdx.x;
@@ -881,7 +885,7 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     const int wv = t >> 6;
     const int64_t p = blockIdx.x;  // the host launches exactly one workgroup per plane
 
-    if (t == 0) irregular[0] = 0;
+    if (t == 0) ndev[0] = 0;
     const double ex = Ex[p];
     for (int r = t; r < NVX; r += 64 * WPP) {
         const Departure d = departure(r, ex, dt, dvx);
@@ -939,11 +943,22 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
     bool pair = false;
     if constexpr (CPL == 2) pair = __ballot(ty[1] != ty[0] + 1) == 0;  // wave-uniform
     __syncthreads();
-    const int s0 = ilx[0];
+    // floor(RN(r - disp)) - r takes at most two values, floor(-disp) and one
+    // more (where r - disp rounds up to an integer: with |disp| below half an
+    // ulp of r, every row but the first ones).  The windowed walks below take
+    // the middle row's offset s0 and load the taps of the few other rows
+    // directly; planes where more rows differ take the per-row path.
+    // Only the float64 and plain float32 kicks of 128 x 128 planes compile
+    // the walk that skips such rows: in the others (deviation storage, 64 rows
+    // or columns) its registers cost a wave per SIMD, so any such row sends
+    // the plane to the per-row path there.
+    constexpr bool WALK_CHECK = !DEV && NVX == 128 && NVY == 128;
+    const int s0 = ilx[NVX / 2] - NVX / 2;
     for (int r = t; r < NVX; r += 64 * WPP)
-        if (ilx[r] - r != s0) irregular[0] = 1;
+        if (ilx[r] - r != s0) atomicAdd(&ndev[0], 1);
     __syncthreads();
-    const bool regular = irregular[0] == 0;  // uniform over the workgroup
+    const int nd = ndev[0];
+    const bool regular = WALK_CHECK ? nd <= kKickMaxDev : nd == 0;  // uniform over the workgroup
 
     const S* src = fin + p * NVX * NVY + CPL * lane;
     S* dst = fout + p * NVX * NVY + CPL * lane;
@@ -1037,7 +1052,12 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
 
     const int r_lo = wv * RPW;
     const bool down = WPP > 1 && (wv & 1) == 0;
-    if (regular && !down) {
+    // CHECK: the plane has rows whose offset is not s0; the walks skip them
+    // (they still feed the window) and the per-row path below computes them.
+    // Without such rows the walks are compiled without the test.
+    auto walk = [&](auto check) {
+    constexpr bool CHECK = decltype(check)::value;
+    if (!down) {
         // rows r_lo .. r_lo + RPW - 1 upwards; window = taps 0..4 of row r
         V2 win[5];
         Raw q[PF];
@@ -1052,18 +1072,19 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 const int jn = r + s0 + 3;  // row held by q[u]: tap 5 of row r
                 const V2 v5 = val(q[u]) * maskr(jn);
                 const V2 v[6] = {win[0], win[1], win[2], win[3], win[4], v5};
+                const bool own = !CHECK || ilx[r] - r == s0;  // uniform; the other rows after the walk
                 C ox0, ox1;
-                vx_pair(r, v, ox0, ox1);
+                if (own) vx_pair(r, v, ox0, ox1);
                 win[0] = win[1];
                 win[1] = win[2];
                 win[2] = win[3];
                 win[3] = win[4];
                 win[4] = v5;
                 if (i0 + u + PF < RPW) q[u] = ld(clampr(jn + PF));
-                vy_row(r, ox0, ox1);
+                if (own) vy_row(r, ox0, ox1);
             }
         }
-    } else if (regular) {
+    } else {
         // rows r_lo + RPW - 1 .. r_lo downwards; window = taps 1..5 of row r
         const int r_hi = r_lo + RPW - 1;
         V2 win[5];
@@ -1079,21 +1100,37 @@ __global__ __launch_bounds__(64 * WPP) void kick_rows_kernel(const S* __restrict
                 const int jn = r + s0 - 2;  // row held by q[u]: tap 0 of row r
                 const V2 v0 = val(q[u]) * maskr(jn);
                 const V2 v[6] = {v0, win[0], win[1], win[2], win[3], win[4]};
+                const bool own = !CHECK || ilx[r] - r == s0;  // uniform; the other rows after the walk
                 C ox0, ox1;
-                vx_pair(r, v, ox0, ox1);
+                if (own) vx_pair(r, v, ox0, ox1);
                 win[4] = win[3];
                 win[3] = win[2];
                 win[2] = win[1];
                 win[1] = win[0];
                 win[0] = v0;
                 if (i0 + u + PF < RPW) q[u] = ld(clampr(jn - PF));
-                vy_row(r, ox0, ox1);
+                if (own) vy_row(r, ox0, ox1);
             }
         }
-    } else {
+    }
+    };
+    if (regular) {
+        if constexpr (WALK_CHECK) {
+            if (nd == 0)
+                walk(std::integral_constant<bool, false>{});
+            else
+                walk(std::integral_constant<bool, true>{});
+        } else {
+            walk(std::integral_constant<bool, false>{});
+        }
+    }
+    // The per-row path: every row of an irregular plane, or after a windowed
+    // walk the few rows whose offset is not s0 (their taps from memory).
+    if (!regular || nd > 0) {  // uniform
         for (int i = 0; i < RPW; ++i) {
             const int r = r_lo + i;
             const int l = ilx[r];
+            if (regular && l - r == s0) continue;  // uniform
             V2 v[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) v[k] = val(ld(clampr(l + k - 2))) * maskr(l + k - 2);

@@ -291,14 +291,16 @@ def test_fp32_kick_kernels_agree(device, shape):
     """The float32 row-split kick (128 vy columns, 2/4/8/16 waves per plane,
     4/8 prefetched rows) and
     the two-wave row-streaming kick give the same bits, with multi-cell and
-    out-of-domain shifts in the field."""
+    out-of-domain shifts in the field and displacements of 8e-17 and 1e-15
+    cells (two and seventeen rows of a plane whose departure rounds differently
+    from the rest)."""
     v2, runner, L = _mods()
     lib = L.lib()
     nx, ny, nvx, nvy = shape
     rng = np.random.default_rng(5)
     f = torch.from_numpy(rng.random(shape)).float().to(device)
     dt, dvx, dvy = 0.1, 12.0 / (nvx - 1), 12.0 / (nvy - 1)
-    choices = np.array([0.0, 1e-18, 0.4, -0.7, 2.0, 5.3, -5.3, 80.0, -80.0])
+    choices = np.array([0.0, 1e-18, 8e-17, 1e-15, 0.4, -0.7, 2.0, 5.3, -5.3, 80.0, -80.0])
     Ex = torch.from_numpy(rng.choice(choices, size=(nx, ny)) * dvx / dt).to(device)
     Ey = torch.from_numpy(rng.choice(choices, size=(nx, ny)) * dvy / dt).to(device)
     sh = L.i64x4(shape)
@@ -408,7 +410,7 @@ def test_deviation_kick_matches_float64_kick_of_the_sum(device, shape):
     fy = np.exp(-0.5 * np.linspace(-4, 4, nvy) ** 2) / 2.5
     E = fx[:, None] * fy[None, :]
     dt, dvx, dvy = 0.1, 8.0 / (nvx - 1), 8.0 / (nvy - 1)
-    choices = np.array([0.0, 1e-18, 0.4, -0.7, 2.0, 5.3, -5.3, 80.0, -80.0])
+    choices = np.array([0.0, 1e-18, 8e-17, 1e-15, 0.4, -0.7, 2.0, 5.3, -5.3, 80.0, -80.0])
     Ex = rng.choice(choices, size=(nx, ny)) * dvx / dt
     Ey = rng.choice(choices, size=(nx, ny)) * dvy / dt
     s = torch.cuda.current_stream().cuda_stream

@@ -160,7 +160,10 @@ def test_kick_stream_and_plane_kernels_agree_on_edge_fields(device, shape):
     8 and 16 waves per plane and 4 or 8 prefetched rows, two-wave
     row-streaming kernel), the in-place whole-plane
     LDS kick and the oracle agree bit for bit on fields that exercise every
-    tap path: E = 0, tiny E (rounding makes floor(r - d) - r non-constant),
+    tap path: E = 0, tiny E (rounding makes floor(r - d) - r non-constant:
+    displacements of 1e-18, 3e-17 and 8e-17 cells move the first one or two
+    rows of a plane, which the windowed walks leave to the per-row path;
+    1e-15 cells moves seventeen, and the whole plane takes the per-row path),
     multi-cell and fully out-of-domain shifts, both signs."""
     v2, runner, L = _mods()
     lib = L.lib()
@@ -168,8 +171,9 @@ def test_kick_stream_and_plane_kernels_agree_on_edge_fields(device, shape):
     rng = np.random.default_rng(11)
     f = rng.random(shape)
     dt, dvx, dvy = 0.1, 12.0 / (nvx - 1), 12.0 / (nvy - 1)
-    choices = np.array([0.0, 1e-18, -1e-18, 3e-17 * dvx / dt, 5.3 * dvx / dt, -5.3 * dvx / dt,
-                        0.4 * dvx / dt, -0.7 * dvx / dt, 80.0 * dvx / dt, -80.0 * dvx / dt, 2.0 * dvx / dt])
+    choices = np.array([0.0, 1e-18, -1e-18, 3e-17 * dvx / dt, 8e-17 * dvx / dt, 1e-15 * dvx / dt, 5.3 * dvx / dt,
+                        -5.3 * dvx / dt, 0.4 * dvx / dt, -0.7 * dvx / dt, 80.0 * dvx / dt, -80.0 * dvx / dt,
+                        2.0 * dvx / dt])
     Ex = rng.choice(choices, size=(nx, ny))
     Ey = rng.choice(choices, size=(nx, ny)) * (dvy / dvx)
     ref = vo.advect_axis(f, Ex[:, :, None], 2, dt, dvx, periodic=False)
