@@ -431,7 +431,8 @@ enum {
                                   at 128 vy columns: 2 = two-wave
                                   row-streaming kick, 3/4/5/6 = row-split
                                   kick with 2/4/8/16 waves per plane
-                                  (0: 8 waves per plane)                       */
+                                  (0: 8 waves per plane for float64, 4 for
+                                  float32)                                     */
     VP_KNOB_DRIFT_DMA = 4,     /* drifts at 64/128 rows: 1 = register prefetch,
                                   2 or 3 = direct-to-LDS ring of that many
                                   tiles (0 = 2)                                */
