@@ -1323,8 +1323,14 @@ static int xdrift(const S* fin, S* fout, const int64_t shape[4], const Fast2DWs&
     const int nq = static_cast<int>(nvy / KC);
     int S_ = knob(VP_KNOB_X_SPLIT);
     if (S_ <= 0) {
+        // workgroups per CU to aim at: 2 (float64; 128^4: 1 range per unit
+        // 786 us for the pair against 794-807 for 2-4); float32 with at least
+        // one unit per CU: 1 for the pair, 4 for a single pass (128^4: pair
+        // 394 us against 403 with 2 ranges; single 437-440 against 467-471)
+        int target = 2;
+        if (sizeof(S) == 4 && nvx * nq >= cu_count()) target = passes == 2 ? 1 : 4;
         S_ = 1;
-        while (nvx * nq * S_ < 2 * cu_count() && S_ < static_cast<int>(ny)) S_ *= 2;
+        while (nvx * nq * S_ < target * cu_count() && S_ < static_cast<int>(ny)) S_ *= 2;
     }
     if (S_ > static_cast<int>(ny)) S_ = static_cast<int>(ny);
     const int64_t a_stride = nvy, row_stride = ny * nvx * nvy, loop_stride = nvx * nvy;
